@@ -1,0 +1,798 @@
+/*
+ * dsem_oracle.c -- CPU oracle (plain C, FP64) for the DSEM staggered-grid
+ * spectral-element RHS and the Carpenter-Kennedy 5-stage RK4 2N step.
+ *
+ * TEST INFRASTRUCTURE ONLY: the checker for tests/, smoke() and the
+ * cpu_baseline leg of bench.py.  Not part of the product library.
+ *
+ * Every routine restates the reference routine cited next to it, in the
+ * reference's floating-point operation order (left-to-right sums, the same
+ * association of products), so that it can be compared bit-for-bit with the
+ * reference routines that compile here (see oracle/ref_harness.f90).  Build
+ * with -ffp-contract=off so that no multiply-add is fused.
+ *
+ * Parity pins (tests/test_oracle_*.py):
+ *   or_ops, or_geometry, or_mortar_geometry, gauss_deriv, contractions,
+ *   or_dtcalc, or_rkcoefs  -> bit-for-bit against the compiled reference
+ *                             leaf routines (oracle/_ref) + closed forms;
+ *   or_euler_flux          -> tensor-form Euler flux (closed form);
+ *   or_riemann             -> textbook eigen-decomposed Roe flux where the
+ *                             entropy fix is inactive, consistency, symmetry;
+ *   or_vis_flux            -> tensor-form Navier-Stokes stress/heat flux;
+ *   or_rhs / or_rk_step    -> free-stream preservation, discrete conservation,
+ *                             spectral convergence to an independent FFT
+ *                             evaluation of the compressible NS RHS, RK4 order.
+ */
+#include "dsem_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#include <float.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define NEQ 5
+static const double PI_REF = 3.141592653589793238462643;
+
+/* ------------------------------------------------------------------ */
+/* Lagrange basis (src/Independentroutines/baselib.f90:18-48, 53-80)   */
+/* ------------------------------------------------------------------ */
+static double polyn(int k, double x, int n, const double *z)
+{
+    /* k is 1-based like the reference */
+    double r;
+    if (k == 1) {
+        r = (x - z[1]) / (z[0] - z[1]);
+        for (int j = 3; j <= n; ++j) r = r * (x - z[j - 1]) / (z[0] - z[j - 1]);
+    } else {
+        r = (x - z[0]) / (z[k - 1] - z[0]);
+        for (int j = 2; j <= k - 1; ++j) r = r * (x - z[j - 1]) / (z[k - 1] - z[j - 1]);
+        for (int j = k + 1; j <= n; ++j) r = r * (x - z[j - 1]) / (z[k - 1] - z[j - 1]);
+    }
+    return r;
+}
+
+static double polynder(int k, double x, int n, const double *z)
+{
+    double hp = 0.0;
+    for (int l = 1; l <= n; ++l) {
+        if (l == k) continue;
+        double poly = 1.0;
+        for (int m = 1; m <= n; ++m) {
+            if (m == l || m == k) continue;
+            poly = poly * (x - z[m - 1]) / (z[k - 1] - z[m - 1]);
+        }
+        hp = hp + poly / (z[k - 1] - z[l - 1]);
+    }
+    return hp;
+}
+
+/* ------------------------------------------------------------------ */
+/* Points and operators                                                */
+/* set_points: src/Grid_Routines/stac3m_geomLib.f90:7-47               */
+/* dermat/intrpmat: src/Spectral/stac3m/matrices.f90:144-207           */
+/* dmx is stored [ncg][nc] (row = Gauss point), bx [nc][ncg].          */
+/* ------------------------------------------------------------------ */
+int or_ops(int p, double *cx, double *cxg, double *dmx, double *bx)
+{
+    if (p < 1 || p > 8) return -1;
+    const int ncg = p + 1, nc = p + 2;
+    const int na = nc - 1;
+    for (int n = 1; n <= nc; ++n) {
+        double xx = cos((n - 1) * PI_REF / na);
+        cx[n - 1] = 0.5 * (1.0 - xx);
+    }
+    for (int n = 1; n <= ncg; ++n) {
+        double xx = cos((2.0 * n - 1.0) * PI_REF / (2.0 * (ncg - 1.0) + 2.0));
+        cxg[n - 1] = 0.5 * (1.0 - xx);
+    }
+    const double eps = DBL_EPSILON;
+    /* dermat(ncol, no=nc, xo=cx, nnew=ncg, xnew=cxg, b): b(k,j), k Gauss, j Lobatto.
+     * The diagonal entry b(k,k) (column k of the Lobatto basis) is replaced by
+     * minus the running sum of the other entries of the row. */
+    for (int k = 1; k <= ncg; ++k) {
+        double sum = 0.0;
+        for (int j = 1; j <= nc; ++j) {
+            if (j == k) continue;
+            double b = polynder(j, cxg[k - 1], nc, cx);
+            if (fabs(b) < eps) b = 0.0;
+            dmx[(k - 1) * nc + (j - 1)] = b;
+            sum = sum - b;
+            dmx[(k - 1) * nc + (k - 1)] = sum;
+        }
+    }
+    /* intrpmat(ncol, no=ncg, xo=cxg, nnew=nc, xnew=cx, b): b(k,j) = h_j(cx_k) */
+    for (int k = 1; k <= nc; ++k)
+        for (int j = 1; j <= ncg; ++j)
+            bx[(k - 1) * ncg + (j - 1)] = polyn(j, cx[k - 1], ncg, cxg);
+    return 0;
+}
+
+/* Carpenter-Kennedy RK4(5) 2N coefficients, baselib.f90:135-152 (iord=4). */
+void or_rkcoefs(double *ark, double *brk, double *crk)
+{
+    ark[0] = 0.0; ark[1] = -0.4178904745; ark[2] = -1.192151694643;
+    ark[3] = -1.697784692471; ark[4] = -1.514183444257;
+    brk[0] = 0.0; brk[1] = 0.1496590219993; brk[2] = 0.3704009573644;
+    brk[3] = 0.6222557631345; brk[4] = 0.9582821306748;
+    crk[0] = 0.1496590219993; crk[1] = 0.3792103129999; crk[2] = 0.8229550293869;
+    crk[3] = 0.6994504559488; crk[4] = 0.1530572479681;
+}
+
+/* ------------------------------------------------------------------ */
+/* hex8 geometry: src/Grid_Routines/GeomLib3D.f90:60-140,685-734       */
+/* corners[8][3] in standard FE order; cx in [0,1]^3.                  */
+/* ------------------------------------------------------------------ */
+static void hex8_map(const double *c, const double *X, double *x)
+{
+    for (int j = 0; j < 3; ++j) {
+        x[j] = c[0 * 3 + j] * (1.0 - X[0]) * (1.0 - X[1]) * (1.0 - X[2])
+             + c[1 * 3 + j] * X[0] * (1.0 - X[1]) * (1.0 - X[2])
+             + c[2 * 3 + j] * X[0] * X[1] * (1.0 - X[2])
+             + c[3 * 3 + j] * (1.0 - X[0]) * X[1] * (1.0 - X[2])
+             + c[4 * 3 + j] * (1.0 - X[0]) * (1.0 - X[1]) * X[2]
+             + c[5 * 3 + j] * X[0] * (1.0 - X[1]) * X[2]
+             + c[6 * 3 + j] * X[0] * X[1] * X[2]
+             + c[7 * 3 + j] * (1.0 - X[0]) * X[1] * X[2];
+    }
+}
+
+/* g[i][j] = d x_i / d X_j */
+static void grad_hex8_map(const double *c, const double *X, double g[3][3])
+{
+    for (int i = 0; i < 3; ++i) {
+        g[i][0] = -(c[0 * 3 + i] * (1.0 - X[1]) * (1.0 - X[2]))
+                + c[1 * 3 + i] * (1.0 - X[1]) * (1.0 - X[2])
+                + c[2 * 3 + i] * X[1] * (1.0 - X[2])
+                - c[3 * 3 + i] * X[1] * (1.0 - X[2])
+                - c[4 * 3 + i] * (1.0 - X[1]) * X[2]
+                + c[5 * 3 + i] * (1.0 - X[1]) * X[2]
+                + c[6 * 3 + i] * X[1] * X[2]
+                - c[7 * 3 + i] * X[1] * X[2];
+        g[i][1] = -(c[0 * 3 + i] * (1.0 - X[0]) * (1.0 - X[2]))
+                - c[1 * 3 + i] * X[0] * (1.0 - X[2])
+                + c[2 * 3 + i] * X[0] * (1.0 - X[2])
+                + c[3 * 3 + i] * (1.0 - X[0]) * (1.0 - X[2])
+                - c[4 * 3 + i] * (1.0 - X[0]) * X[2]
+                - c[5 * 3 + i] * X[0] * X[2]
+                + c[6 * 3 + i] * X[0] * X[2]
+                + c[7 * 3 + i] * (1.0 - X[0]) * X[2];
+        g[i][2] = -(c[0 * 3 + i] * (1.0 - X[0]) * (1.0 - X[1]))
+                - c[1 * 3 + i] * X[0] * (1.0 - X[1])
+                - c[2 * 3 + i] * X[0] * X[1]
+                - c[3 * 3 + i] * (1.0 - X[0]) * X[1]
+                + c[4 * 3 + i] * (1.0 - X[0]) * (1.0 - X[1])
+                + c[5 * 3 + i] * X[0] * (1.0 - X[1])
+                + c[6 * 3 + i] * X[0] * X[1]
+                + c[7 * 3 + i] * (1.0 - X[0]) * X[1];
+    }
+}
+
+/* gi[i][j] = d X_i / d x_j, returns J = det(g) */
+static double invert_metric(const double g[3][3], double gi[3][3])
+{
+    const double small = 100.0 * DBL_EPSILON;
+    double J = g[0][0] * g[1][1] * g[2][2]
+             - g[0][1] * g[1][0] * g[2][2]
+             - g[0][0] * g[1][2] * g[2][1]
+             + g[0][2] * g[1][0] * g[2][1]
+             + g[0][1] * g[1][2] * g[2][0]
+             - g[0][2] * g[1][1] * g[2][0];
+    gi[0][0] = (g[1][1] * g[2][2] - g[1][2] * g[2][1]) / J;
+    gi[1][0] = (-(g[1][0] * g[2][2]) + g[1][2] * g[2][0]) / J;
+    gi[2][0] = (g[1][0] * g[2][1] - g[2][0] * g[1][1]) / J;
+    gi[0][1] = (-(g[0][1] * g[2][2]) + g[0][2] * g[2][1]) / J;
+    gi[1][1] = (g[0][0] * g[2][2] - g[0][2] * g[2][0]) / J;
+    gi[2][1] = (-(g[0][0] * g[2][1]) + g[0][1] * g[2][0]) / J;
+    gi[0][2] = (g[0][1] * g[1][2] - g[1][1] * g[0][2]) / J;
+    gi[1][2] = (-(g[0][0] * g[1][2]) + g[0][2] * g[1][0]) / J;
+    gi[2][2] = (g[0][0] * g[1][1] - g[0][1] * g[1][0]) / J;
+    for (int m = 0; m < 3; ++m)
+        for (int n = 0; n < 3; ++n)
+            if (fabs(gi[n][m]) <= small) gi[n][m] = 0.0;
+    return J;
+}
+
+/* set_geometry, all-straight hex8 branch: set_domain_geometry.f90:77-145 */
+int or_geometry(int K, int p, const double *corners, double *xg, double *jacob,
+                double *gmetg, double *gmetx, double *gmety, double *gmetz)
+{
+    double cx[10], cxg[9], dmx[90], bx[90];
+    if (or_ops(p, cx, cxg, dmx, bx)) return -1;
+    const int ncg = p + 1, nc = p + 2, np = ncg * ncg * ncg, ns = nc * ncg * ncg;
+#pragma omp parallel for schedule(static)
+    for (int e = 0; e < K; ++e) {
+        const double *c = corners + (size_t)e * 24;
+        double X[3], g[3][3], gi[3][3];
+        for (int l = 0; l < ncg; ++l)
+            for (int m = 0; m < ncg; ++m)
+                for (int n = 0; n < ncg; ++n) {
+                    const int pt = n + ncg * (m + ncg * l);
+                    X[0] = cxg[n]; X[1] = cxg[m]; X[2] = cxg[l];
+                    if (xg) hex8_map(c, X, xg + ((size_t)e * np + pt) * 3);
+                    grad_hex8_map(c, X, g);
+                    double J = invert_metric(g, gi);
+                    jacob[(size_t)e * np + pt] = 1.0 / J;
+                    for (int a = 0; a < 3; ++a)
+                        for (int b = 0; b < 3; ++b)
+                            gmetg[((size_t)e * np + pt) * 9 + a * 3 + b] = gi[a][b];
+                }
+        for (int l = 0; l < ncg; ++l)
+            for (int m = 0; m < ncg; ++m)
+                for (int j = 0; j < nc; ++j) {
+                    X[0] = cx[j]; X[1] = cxg[m]; X[2] = cxg[l];
+                    grad_hex8_map(c, X, g);
+                    double J = invert_metric(g, gi);
+                    double *o = gmetx + ((size_t)e * ns + j + nc * (m + ncg * l)) * 3;
+                    for (int b = 0; b < 3; ++b) o[b] = J * gi[0][b];
+                }
+        for (int l = 0; l < ncg; ++l)
+            for (int j = 0; j < nc; ++j)
+                for (int n = 0; n < ncg; ++n) {
+                    X[0] = cxg[n]; X[1] = cx[j]; X[2] = cxg[l];
+                    grad_hex8_map(c, X, g);
+                    double J = invert_metric(g, gi);
+                    double *o = gmety + ((size_t)e * ns + n + ncg * (j + nc * l)) * 3;
+                    for (int b = 0; b < 3; ++b) o[b] = J * gi[1][b];
+                }
+        for (int j = 0; j < nc; ++j)
+            for (int m = 0; m < ncg; ++m)
+                for (int n = 0; n < ncg; ++n) {
+                    X[0] = cxg[n]; X[1] = cxg[m]; X[2] = cx[j];
+                    grad_hex8_map(c, X, g);
+                    double J = invert_metric(g, gi);
+                    double *o = gmetz + ((size_t)e * ns + n + ncg * (m + ncg * j)) * 3;
+                    for (int b = 0; b < 3; ++b) o[b] = J * gi[2][b];
+                }
+    }
+    return 0;
+}
+
+/* Mortar normals from the master face: set_domain_geometry.f90:254-345.
+ * end_vals = (0,1,0,1,1,0) for faces 1..6. */
+int or_mortar_geometry(int M, int p, const double *corners, const int32_t *m_elem,
+                       const int32_t *m_face, double *nhat, double *norm)
+{
+    double cx[10], cxg[9], dmx[90], bx[90];
+    if (or_ops(p, cx, cxg, dmx, bx)) return -1;
+    static const double end_vals[6] = {0.0, 1.0, 0.0, 1.0, 1.0, 0.0};
+    const int ncg = p + 1, nf = ncg * ncg;
+#pragma omp parallel for schedule(static)
+    for (int mm = 0; mm < M; ++mm) {
+        const int e = m_elem[2 * mm], f = m_face[2 * mm];
+        const double *c = corners + (size_t)e * 24;
+        double X[3], g[3][3], gi[3][3];
+        int row, a0, a1, fixed;
+        if (f == 1 || f == 2) { fixed = 1; a0 = 0; a1 = 2; row = 1; }
+        else if (f == 3 || f == 5) { fixed = 2; a0 = 0; a1 = 1; row = 2; }
+        else { fixed = 0; a0 = 1; a1 = 2; row = 0; }
+        for (int j = 0; j < ncg; ++j)
+            for (int i = 0; i < ncg; ++i) {
+                X[fixed] = end_vals[f - 1];
+                X[a0] = cxg[i];
+                X[a1] = cxg[j];
+                grad_hex8_map(c, X, g);
+                double J = invert_metric(g, gi);
+                double sqr = sqrt(gi[row][0] * gi[row][0] + gi[row][1] * gi[row][1]
+                                  + gi[row][2] * gi[row][2]);
+                double *nh = nhat + ((size_t)mm * nf + i + ncg * j) * 3;
+                nh[0] = gi[row][0] / sqr;
+                nh[1] = gi[row][1] / sqr;
+                nh[2] = gi[row][2] / sqr;
+                norm[(size_t)mm * nf + i + ncg * j] = J * sqr;
+            }
+    }
+    return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* Pointwise physics: src/Parabolic3D/NS/NS/PhysLib.f90                */
+/* ------------------------------------------------------------------ */
+/* fflux/gflux/hflux: PhysLib.f90:17-124 */
+void or_euler_flux(const or_phys *ph, const double *Q, double *f, double *g, double *h)
+{
+    const double gamma = ph->gamma;
+    const double rho = Q[0], rhou = Q[1], rhov = Q[2], rhow = Q[3], rhoe = Q[4];
+    const double u = rhou / rho, v = rhov / rho, w = rhow / rho;
+    const double p = (gamma - 1.0) * (rhoe - 0.5 * rho * (u * u + v * v + w * w));
+    if (f) {
+        f[0] = rhou; f[1] = p + rhou * u; f[2] = rhou * v; f[3] = rhou * w; f[4] = u * (rhoe + p);
+    }
+    if (g) {
+        g[0] = rhov; g[1] = rhou * v; g[2] = p + rhov * v; g[3] = rhow * v; g[4] = v * (rhoe + p);
+    }
+    if (h) {
+        h[0] = rhow; h[1] = rhou * w; h[2] = rhov * w; h[3] = p + rhow * w; h[4] = w * (rhoe + p);
+    }
+}
+
+/* Roe flux with entropy fix: PhysLib.f90:231-320 */
+void or_riemann(const or_phys *ph, const double *nhat, double ds, const double *Ql,
+                const double *Qr, double *flux)
+{
+    const double gamma = ph->gamma;
+    const double rho = Ql[0], rhou = Ql[1], rhov = Ql[2], rhow = Ql[3], rhoe = Ql[4];
+    const double rhon = Qr[0], rhoun = Qr[1], rhovn = Qr[2], rhown = Qr[3], rhoen = Qr[4];
+    const double ul = rhou / rho, vl = rhov / rho, wl = rhow / rho;
+    const double pleft = (gamma - 1.0) * (rhoe - 0.5 / rho * (rhou * rhou + rhov * rhov + rhow * rhow));
+    const double ur = rhoun / rhon, vr = rhovn / rhon, wr = rhown / rhon;
+    const double pright = (gamma - 1.0) * (rhoen - 0.5 / rhon * (rhoun * rhoun + rhovn * rhovn + rhown * rhown));
+    const double ql = nhat[0] * ul + nhat[1] * vl + nhat[2] * wl;
+    const double qr = nhat[0] * ur + nhat[1] * vr + nhat[2] * wr;
+    const double hl = 0.5 * (ul * ul + vl * vl + wl * wl) + gamma / (gamma - 1.0) * pleft / rho;
+    const double hr = 0.5 * (ur * ur + vr * vr + wr * wr) + gamma / (gamma - 1.0) * pright / rhon;
+    const double rtd = sqrt(rho * rhon);
+    const double betal = rho / (rho + rtd);
+    const double betar = 1.0 - betal;
+    const double utd = betal * ul + betar * ur;
+    const double vtd = betal * vl + betar * vr;
+    const double wtd = betal * wl + betar * wr;
+    const double htd = betal * hl + betar * hr;
+    const double atd2 = (gamma - 1.0) * (htd - 0.5 * (utd * utd + vtd * vtd + wtd * wtd));
+    const double atd = sqrt(atd2);
+    const double qtd = utd * nhat[0] + vtd * nhat[1] + wtd * nhat[2];
+    if (qtd >= 0.0) {
+        const double dw1 = 0.5 * ((pright - pleft) / atd2 - (qr - ql) * rtd / atd);
+        const double sp1 = qtd - atd;
+        const double sp1m = fmin(sp1, 0.0);
+        const double hd1m = ((gamma + 1.0) / 4.0 * atd / rtd) * dw1;
+        const double eta1 = fmax(-fabs(sp1) - hd1m, 0.0);
+        const double udw1 = dw1 * (sp1m - 0.5 * eta1);
+        const double rql = rho * ql;
+        flux[0] = ds * (rql + udw1);
+        flux[1] = ds * (rql * ul + pleft * nhat[0] + udw1 * (utd - atd * nhat[0]));
+        flux[2] = ds * (rql * vl + pleft * nhat[1] + udw1 * (vtd - atd * nhat[1]));
+        flux[3] = ds * (rql * wl + pleft * nhat[2] + udw1 * (wtd - atd * nhat[2]));
+        flux[4] = ds * (rql * hl + udw1 * (htd - qtd * atd));
+    } else {
+        const double dw4 = 0.5 * ((pright - pleft) / atd2 + (qr - ql) * rtd / atd);
+        const double sp4 = qtd + atd;
+        const double sp4p = fmax(sp4, 0.0);
+        const double hd4 = ((gamma + 1.0) / 4.0 * atd / rtd) * dw4;
+        const double eta4 = fmax(-fabs(sp4) + hd4, 0.0);
+        const double udw4 = dw4 * (sp4p + 0.5 * eta4);
+        const double rqr = rhon * qr;
+        flux[0] = ds * (rqr - udw4);
+        flux[1] = ds * (rqr * ur + pright * nhat[0] - udw4 * (utd + atd * nhat[0]));
+        flux[2] = ds * (rqr * vr + pright * nhat[1] - udw4 * (vtd + atd * nhat[1]));
+        flux[3] = ds * (rqr * wr + pright * nhat[2] - udw4 * (wtd + atd * nhat[2]));
+        flux[4] = ds * (rqr * hr - udw4 * (htd + qtd * atd));
+    }
+}
+
+/* x/y/z_vis_flux: PhysLib.f90:419-583 (each recomputes the same primitives). */
+void or_vis_flux(const or_phys *ph, const double *Q, const double *Qx, const double *Qy,
+                 const double *Qz, double amu, double *f, double *g, double *h)
+{
+    const double gamma = ph->gamma, mach = ph->mach, pr = ph->pr;
+    const double rho = Q[0], rhou = Q[1], rhov = Q[2], rhow = Q[3], rhoe = Q[4];
+    const double u = rhou / rho, v = rhov / rho, w = rhow / rho;
+    const double p = (gamma - 1.0) * (rhoe - 0.5 * rho * (u * u + v * v + w * w));
+    const double ux = (Qx[1] - u * Qx[0]) / rho, uy = (Qy[1] - u * Qy[0]) / rho, uz = (Qz[1] - u * Qz[0]) / rho;
+    const double vx = (Qx[2] - v * Qx[0]) / rho, vy = (Qy[2] - v * Qy[0]) / rho, vz = (Qz[2] - v * Qz[0]) / rho;
+    const double wx = (Qx[3] - w * Qx[0]) / rho, wy = (Qy[3] - w * Qy[0]) / rho, wz = (Qz[3] - w * Qz[0]) / rho;
+    const double akappa = 1.0 + (amu - 1.0) * 2.0e-1 / (gamma - 1.0);
+    const double div = ux + vy + wz;
+    const double cden = pr * mach * mach * (gamma - 1.0);
+    if (f) {
+        const double px = (gamma - 1.0) * (Qx[4] - 0.5 * Qx[0] * (u * u + v * v + w * w) - rho * (u * ux + v * vx + w * wx));
+        const double tx = (px - p * Qx[0] / rho) * mach * mach * gamma / rho;
+        f[0] = 0.0;
+        f[1] = 2.0 * amu * (ux - div / 3.0);
+        f[2] = amu * (uy + vx);
+        f[3] = amu * (wx + uz);
+        f[4] = u * f[1] + v * f[2] + w * f[3] + akappa * tx / cden;
+    }
+    if (g) {
+        const double py = (gamma - 1.0) * (Qy[4] - 0.5 * Qy[0] * (u * u + v * v + w * w) - rho * (u * uy + v * vy + w * wy));
+        const double ty = (py - p * Qy[0] / rho) * mach * mach * gamma / rho;
+        g[0] = 0.0;
+        g[1] = amu * (uy + vx);
+        g[2] = 2.0 * amu * (vy - div / 3.0);
+        g[3] = amu * (wy + vz);
+        g[4] = v * g[2] + u * g[1] + w * g[3] + akappa * ty / cden;
+    }
+    if (h) {
+        const double pz = (gamma - 1.0) * (Qz[4] - 0.5 * Qz[0] * (u * u + v * v + w * w) - rho * (u * uz + v * vz + w * wz));
+        const double tz = (pz - p * Qz[0] / rho) * mach * mach * gamma / rho;
+        h[0] = 0.0;
+        h[1] = amu * (wx + uz);
+        h[2] = amu * (wy + vz);
+        h[3] = 2.0 * amu * (wz - div / 3.0);
+        h[4] = v * h[2] + u * h[1] + w * h[3] + akappa * tz / cden;
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* Face slots: which staggered-grid entry holds face point (i,j).       */
+/* send_q.f90:21-280, send_flux.f90:22-236                             */
+/* ------------------------------------------------------------------ */
+static void face_slot(int face, int i, int j, int ncg, int nc, int *grid, int *idx)
+{
+    switch (face) {
+    case 1: *grid = 1; *idx = i + ncg * (0 + nc * j); break;
+    case 2: *grid = 1; *idx = i + ncg * ((nc - 1) + nc * j); break;
+    case 3: *grid = 2; *idx = i + ncg * (j + ncg * 0); break;
+    case 5: *grid = 2; *idx = i + ncg * (j + ncg * (nc - 1)); break;
+    case 4: *grid = 0; *idx = (nc - 1) + nc * (i + ncg * j); break;
+    default: *grid = 0; *idx = 0 + nc * (i + ncg * j); break; /* face 6 */
+    }
+}
+
+/* Staggered index of Lobatto position j on the line (a,b) of grid d. */
+static inline int sidx(int d, int j, int a, int b, int ncg, int nc)
+{
+    if (d == 0) return j + nc * (a + ncg * b);         /* (j, m=a, l=b) */
+    if (d == 1) return a + ncg * (j + nc * b);         /* (n=a, j, l=b) */
+    return a + ncg * (b + ncg * j);                    /* (n=a, m=b, j) */
+}
+/* Gauss index of position k on line (a,b) in direction d. */
+static inline int gidx(int d, int k, int a, int b, int ncg)
+{
+    if (d == 0) return k + ncg * (a + ncg * b);
+    if (d == 1) return a + ncg * (k + ncg * b);
+    return a + ncg * (b + ncg * k);
+}
+
+typedef struct {
+    int p, ncg, nc, np, ns, nf;
+    double cx[10], cxg[9], dmx[90], bx[90];
+} ops_t;
+
+static int make_ops(int p, ops_t *o)
+{
+    o->p = p; o->ncg = p + 1; o->nc = p + 2;
+    o->np = o->ncg * o->ncg * o->ncg; o->ns = o->nc * o->ncg * o->ncg; o->nf = o->ncg * o->ncg;
+    return or_ops(p, o->cx, o->cxg, o->dmx, o->bx);
+}
+
+/*
+ * The RHS.  Order of operations follows Fluxes (fluxesH_newa.f90:17-199):
+ *   Int_fluxes per element (fluxesH_newa.f90:221-355)
+ *   send_q_to_mortars -> per mortar: interface_flux/riemann -> send_flux_to_faces,
+ *   V_Avg_Soln -> send_q_to_faces  (mortar_operations.f90:87-238,781-837)
+ *   Vis_Fluxes (Vis_Fluxes_new.f90:16-382) with Gauss_Deriv (SolnDerivsc.f90:14-83)
+ *   send_flux_to_mortars, mortar_vis_flux (ViscousBC_slide.f90:231-271),
+ *   send_flux_to_faces_add (send_flux.f90:699-921)
+ *   update_interior divergence (updatea.f90:100-113).
+ * muh: optional entropy viscosity [K][3][ns] on the three staggered grids.
+ */
+int or_rhs(const or_mesh *mesh, const or_phys *ph, const double *JQ, const double *muh,
+           double *Qt, or_debug *dbg, int nthreads)
+{
+    ops_t o;
+    if (make_ops(mesh->p, &o)) return -1;
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#else
+    (void)nthreads;
+#endif
+    const int K = mesh->K, ncg = o.ncg, nc = o.nc, np = o.np, ns = o.ns, nf = o.nf;
+    const double re = ph->re;
+    const size_t sz_stag = (size_t)K * 3 * NEQ * ns;
+    double *S = dbg && dbg->stag ? dbg->stag : (double *)malloc(sizeof(double) * sz_stag);
+    double *F = dbg && dbg->flux ? dbg->flux : (double *)malloc(sizeof(double) * sz_stag);
+    double *R = dbg && dbg->visc ? dbg->visc : (double *)malloc(sizeof(double) * sz_stag);
+    double *Gd = dbg && dbg->grad ? dbg->grad : (double *)malloc(sizeof(double) * (size_t)K * 3 * NEQ * np);
+    double *mQ = (double *)malloc(sizeof(double) * (size_t)mesh->nmort * 2 * NEQ * nf);
+    double *mFV = (double *)malloc(sizeof(double) * (size_t)mesh->nmort * 2 * NEQ * nf);
+    if (!S || !F || !R || !Gd || !mQ || !mFV) return -2;
+    memset(F, 0, sizeof(double) * sz_stag);
+    const double *gm[3] = {mesh->gmetx, mesh->gmety, mesh->gmetz};
+
+    /* ---------------- Int_fluxes ---------------- */
+#pragma omp parallel for schedule(static)
+    for (int e = 0; e < K; ++e) {
+        double Qp[NEQ * 729];
+        const double *jq = JQ + (size_t)e * NEQ * np;
+        const double *jac = mesh->jacob + (size_t)e * np;
+        for (int v = 0; v < NEQ; ++v)
+            for (int pt = 0; pt < np; ++pt) Qp[v * np + pt] = jq[v * np + pt] * jac[pt];
+        for (int d = 0; d < 3; ++d) {
+            double *Sd = S + ((size_t)e * 3 + d) * NEQ * ns;
+            double *Fd = F + ((size_t)e * 3 + d) * NEQ * ns;
+            /* interpxa/ya/za: FInterpLiba.f90:174-275; mxm left-to-right sums */
+            for (int v = 0; v < NEQ; ++v)
+                for (int b = 0; b < ncg; ++b)
+                    for (int a = 0; a < ncg; ++a)
+                        for (int j = 0; j < nc; ++j) {
+                            double s = 0.0;
+                            for (int k = 0; k < ncg; ++k) {
+                                double t = o.bx[j * ncg + k] * Qp[v * np + gidx(d, k, a, b, ncg)];
+                                s = (k == 0) ? t : s + t;
+                            }
+                            Sd[v * ns + sidx(d, j, a, b, ncg, nc)] = s;
+                        }
+            /* interior Lobatto points: contravariant Euler flux */
+            for (int b = 0; b < ncg; ++b)
+                for (int a = 0; a < ncg; ++a)
+                    for (int j = 1; j < nc - 1; ++j) {
+                        const int si = sidx(d, j, a, b, ncg, nc);
+                        double Qv[NEQ], ff[NEQ], gg[NEQ], hh[NEQ];
+                        for (int v = 0; v < NEQ; ++v) Qv[v] = Sd[v * ns + si];
+                        or_euler_flux(ph, Qv, ff, gg, hh);
+                        const double *gmt = gm[d] + ((size_t)e * ns + si) * 3;
+                        for (int v = 0; v < NEQ; ++v)
+                            Fd[v * ns + si] = gmt[0] * ff[v] + gmt[1] * gg[v] + gmt[2] * hh[v];
+                    }
+        }
+    }
+
+    /* ---------------- send_q_to_mortars ---------------- */
+#pragma omp parallel for schedule(static)
+    for (int mm = 0; mm < mesh->nmort; ++mm) {
+        for (int side = 0; side < 2; ++side) {
+            const int e = mesh->m_elem[2 * mm + side], f = mesh->m_face[2 * mm + side];
+            for (int j = 0; j < ncg; ++j)
+                for (int i = 0; i < ncg; ++i) {
+                    int grid, idx;
+                    face_slot(f, i, j, ncg, nc, &grid, &idx);
+                    for (int v = 0; v < NEQ; ++v)
+                        mQ[(((size_t)mm * 2 + side) * NEQ + v) * nf + i + ncg * j] =
+                            S[(((size_t)e * 3 + grid) * NEQ + v) * ns + idx];
+                }
+        }
+    }
+
+    /* ---------------- mortar fluxes + solution average ---------------- */
+#pragma omp parallel for schedule(static)
+    for (int mm = 0; mm < mesh->nmort; ++mm) {
+        const int em = mesh->m_elem[2 * mm], fm = mesh->m_face[2 * mm];
+        const int es = mesh->m_elem[2 * mm + 1], fs = mesh->m_face[2 * mm + 1];
+        const double sgn = (double)mesh->m_sign[mm];
+        for (int j = 0; j < ncg; ++j)
+            for (int i = 0; i < ncg; ++i) {
+                const int q = i + ncg * j;
+                double Qm[NEQ], Qs[NEQ], fl[NEQ], avg[NEQ];
+                for (int v = 0; v < NEQ; ++v) {
+                    Qm[v] = mQ[(((size_t)mm * 2 + 0) * NEQ + v) * nf + q];
+                    Qs[v] = mQ[(((size_t)mm * 2 + 1) * NEQ + v) * nf + q];
+                }
+                const double *nh = mesh->nhat + ((size_t)mm * nf + q) * 3;
+                const double ds = mesh->norm[(size_t)mm * nf + q];
+                if (fm == 1 || fm == 3 || fm == 6) or_riemann(ph, nh, ds, Qs, Qm, fl);
+                else or_riemann(ph, nh, ds, Qm, Qs, fl);
+                for (int v = 0; v < NEQ; ++v) avg[v] = (Qm[v] + Qs[v]) * 0.5;
+                int grid, idx;
+                face_slot(fm, i, j, ncg, nc, &grid, &idx);
+                for (int v = 0; v < NEQ; ++v) {
+                    F[(((size_t)em * 3 + grid) * NEQ + v) * ns + idx] = fl[v];
+                    S[(((size_t)em * 3 + grid) * NEQ + v) * ns + idx] = avg[v];
+                }
+                face_slot(fs, i, j, ncg, nc, &grid, &idx);
+                for (int v = 0; v < NEQ; ++v) {
+                    F[(((size_t)es * 3 + grid) * NEQ + v) * ns + idx] = sgn * fl[v];
+                    S[(((size_t)es * 3 + grid) * NEQ + v) * ns + idx] = avg[v];
+                }
+            }
+    }
+
+    /* ---------------- Vis_Fluxes: per element ---------------- */
+#pragma omp parallel for schedule(static)
+    for (int e = 0; e < K; ++e) {
+        double W[3][NEQ * 729];           /* w1x, w2y, w3z per variable */
+        double *Ge = Gd + (size_t)e * 3 * NEQ * np;
+        const double *gmg = mesh->gmetg + (size_t)e * np * 9;
+        /* Gauss_Deriv: dx3dga/dy3dga/dz3dga (FDerivLibb.f90:153-241) */
+        for (int d = 0; d < 3; ++d) {
+            const double *Sd = S + ((size_t)e * 3 + d) * NEQ * ns;
+            for (int v = 0; v < NEQ; ++v)
+                for (int b = 0; b < ncg; ++b)
+                    for (int a = 0; a < ncg; ++a)
+                        for (int k = 0; k < ncg; ++k) {
+                            double s = 0.0;
+                            for (int j = 0; j < nc; ++j) {
+                                double t = o.dmx[k * nc + j] * Sd[v * ns + sidx(d, j, a, b, ncg, nc)];
+                                s = (j == 0) ? t : s + t;
+                            }
+                            W[d][v * np + gidx(d, k, a, b, ncg)] = s;
+                        }
+        }
+        for (int v = 0; v < NEQ; ++v)
+            for (int pt = 0; pt < np; ++pt) {
+                const double *G9 = gmg + (size_t)pt * 9;   /* G9[i*3+j] = gmetg(i+1,j+1) */
+                const double w1 = W[0][v * np + pt], w2 = W[1][v * np + pt], w3 = W[2][v * np + pt];
+                Ge[(0 * NEQ + v) * np + pt] = G9[0 * 3 + 0] * w1 + G9[1 * 3 + 0] * w2 + G9[2 * 3 + 0] * w3;
+                Ge[(1 * NEQ + v) * np + pt] = G9[0 * 3 + 1] * w1 + G9[1 * 3 + 1] * w2 + G9[2 * 3 + 1] * w3;
+                Ge[(2 * NEQ + v) * np + pt] = G9[0 * 3 + 2] * w1 + G9[1 * 3 + 2] * w2 + G9[2 * 3 + 2] * w3;
+            }
+        /* prolong the 15 gradient fields (interp -> mxv) and viscous fluxes */
+        for (int d = 0; d < 3; ++d) {
+            const double *Sd = S + ((size_t)e * 3 + d) * NEQ * ns;
+            double *Rd = R + ((size_t)e * 3 + d) * NEQ * ns;
+            double *Fd = F + ((size_t)e * 3 + d) * NEQ * ns;
+            const double *gmt = gm[d] + (size_t)e * ns * 3;
+            for (int b = 0; b < ncg; ++b)
+                for (int a = 0; a < ncg; ++a)
+                    for (int j = 0; j < nc; ++j) {
+                        const int si = sidx(d, j, a, b, ncg, nc);
+                        double Qv[NEQ], Gx[3][NEQ], ff[NEQ], gg[NEQ], hh[NEQ];
+                        for (int v = 0; v < NEQ; ++v) Qv[v] = Sd[v * ns + si];
+                        for (int c = 0; c < 3; ++c)
+                            for (int v = 0; v < NEQ; ++v) {
+                                double s = 0.0;
+                                for (int k = 0; k < ncg; ++k) {
+                                    double t = o.bx[j * ncg + k] * Ge[(c * NEQ + v) * np + gidx(d, k, a, b, ncg)];
+                                    s = (k == 0) ? t : s + t;
+                                }
+                                Gx[c][v] = s;
+                            }
+                        double amu = 1.0;
+                        if (muh) amu = amu + muh[((size_t)e * 3 + d) * ns + si];
+                        or_vis_flux(ph, Qv, Gx[0], Gx[1], Gx[2], amu, ff, gg, hh);
+                        const double *gq = gmt + (size_t)si * 3;
+                        for (int v = 0; v < NEQ; ++v)
+                            Rd[v * ns + si] = gq[0] * ff[v] + gq[1] * gg[v] + gq[2] * hh[v];
+                    }
+            /* interior: f = f - r/re (Vis_Fluxes_new.f90:195-228) */
+            for (int v = 0; v < NEQ; ++v)
+                for (int b = 0; b < ncg; ++b)
+                    for (int a = 0; a < ncg; ++a)
+                        for (int j = 1; j < nc - 1; ++j) {
+                            const int si = sidx(d, j, a, b, ncg, nc);
+                            Fd[v * ns + si] = Fd[v * ns + si] - Rd[v * ns + si] / re;
+                        }
+        }
+    }
+
+    /* ---------------- send_flux_to_mortars: fv = r/norm ---------------- */
+#pragma omp parallel for schedule(static)
+    for (int mm = 0; mm < mesh->nmort; ++mm) {
+        for (int side = 0; side < 2; ++side) {
+            const int e = mesh->m_elem[2 * mm + side], f = mesh->m_face[2 * mm + side];
+            for (int j = 0; j < ncg; ++j)
+                for (int i = 0; i < ncg; ++i) {
+                    int grid, idx;
+                    face_slot(f, i, j, ncg, nc, &grid, &idx);
+                    const double nrm = mesh->norm[(size_t)mm * nf + i + ncg * j];
+                    for (int v = 0; v < NEQ; ++v)
+                        mFV[(((size_t)mm * 2 + side) * NEQ + v) * nf + i + ncg * j] =
+                            R[(((size_t)e * 3 + grid) * NEQ + v) * ns + idx] / nrm;
+                }
+        }
+    }
+
+    /* ---------------- mortar_vis_flux + send_flux_to_faces_add ---------------- */
+#pragma omp parallel for schedule(static)
+    for (int mm = 0; mm < mesh->nmort; ++mm) {
+        const int em = mesh->m_elem[2 * mm], fm = mesh->m_face[2 * mm];
+        const int es = mesh->m_elem[2 * mm + 1], fs = mesh->m_face[2 * mm + 1];
+        for (int j = 0; j < ncg; ++j)
+            for (int i = 0; i < ncg; ++i) {
+                const int q = i + ncg * j;
+                const double nrm = mesh->norm[(size_t)mm * nf + q];
+                double Qb[NEQ];
+                for (int v = 0; v < NEQ; ++v) {
+                    double Qs = mFV[(((size_t)mm * 2 + 1) * NEQ + v) * nf + q];
+                    if (mesh->m_nsign[mm] == -1) Qs = -Qs;
+                    const double Qm = mFV[(((size_t)mm * 2 + 0) * NEQ + v) * nf + q];
+                    Qb[v] = 0.5 * (Qm + Qs) * nrm;
+                }
+                int grid, idx;
+                face_slot(fm, i, j, ncg, nc, &grid, &idx);
+                for (int v = 0; v < NEQ; ++v) {
+                    double *t = &F[(((size_t)em * 3 + grid) * NEQ + v) * ns + idx];
+                    *t = *t - Qb[v] / re;
+                }
+                face_slot(fs, i, j, ncg, nc, &grid, &idx);
+                for (int v = 0; v < NEQ; ++v) {
+                    const double fl = (mesh->m_sign[mm] == -1) ? -Qb[v] : Qb[v];
+                    double *t = &F[(((size_t)es * 3 + grid) * NEQ + v) * ns + idx];
+                    *t = *t - fl / re;
+                }
+            }
+    }
+
+    /* ---------------- update_interior: divergence ---------------- */
+#pragma omp parallel for schedule(static)
+    for (int e = 0; e < K; ++e) {
+        double Dv[3][729];
+        for (int v = 0; v < NEQ; ++v) {
+            for (int d = 0; d < 3; ++d) {
+                const double *Fd = F + (((size_t)e * 3 + d) * NEQ + v) * ns;
+                for (int b = 0; b < ncg; ++b)
+                    for (int a = 0; a < ncg; ++a)
+                        for (int k = 0; k < ncg; ++k) {
+                            double s = 0.0;
+                            for (int j = 0; j < nc; ++j) {
+                                double t = o.dmx[k * nc + j] * Fd[sidx(d, j, a, b, ncg, nc)];
+                                s = (j == 0) ? t : s + t;
+                            }
+                            Dv[d][gidx(d, k, a, b, ncg)] = s;
+                        }
+            }
+            double *qt = Qt + ((size_t)e * NEQ + v) * np;
+            for (int pt = 0; pt < np; ++pt) qt[pt] = -(Dv[0][pt] + Dv[1][pt] + Dv[2][pt]);
+        }
+    }
+
+    if (!(dbg && dbg->stag)) free(S);
+    if (!(dbg && dbg->flux)) free(F);
+    if (!(dbg && dbg->visc)) free(R);
+    if (!(dbg && dbg->grad)) free(Gd);
+    free(mQ);
+    free(mFV);
+    return 0;
+}
+
+/*
+ * One time step: tstep (src/tstep.f90:12-120) with update_interior's RK
+ * register update (updatea.f90:213-227).  G is zeroed at the start of the
+ * step.  resid follows the reference: update_interior resets it per element,
+ * so the value left is max|Qt| over the last element of the last stage.
+ */
+int or_rk_step(const or_mesh *mesh, const or_phys *ph, double *JQ, double *G, double dt,
+               const double *muh, double *resid, int nthreads)
+{
+    double ark[5], brk[5], crk[5];
+    or_rkcoefs(ark, brk, crk);
+    const int np = (mesh->p + 1) * (mesh->p + 1) * (mesh->p + 1);
+    const size_t n = (size_t)mesh->K * NEQ * np;
+    double *Qt = (double *)malloc(sizeof(double) * n);
+    if (!Qt) return -2;
+    memset(G, 0, sizeof(double) * n);
+    double last = 0.0;
+    for (int kk = 0; kk < 5; ++kk) {
+        int rc = or_rhs(mesh, ph, JQ, muh, Qt, NULL, nthreads);
+        if (rc) { free(Qt); return rc; }
+        const double a = ark[kk], c = crk[kk];
+#pragma omp parallel for schedule(static)
+        for (int e = 0; e < mesh->K; ++e) {
+            for (int v = 0; v < NEQ; ++v)
+                for (int pt = 0; pt < np; ++pt) {
+                    const size_t i = ((size_t)e * NEQ + v) * np + pt;
+                    G[i] = a * G[i] + Qt[i];
+                    JQ[i] = JQ[i] + c * dt * G[i];
+                }
+        }
+        last = 0.0;
+        const double *q = Qt + (size_t)(mesh->K - 1) * NEQ * np;
+        for (int i = 0; i < NEQ * np; ++i) last = fmax(fabs(q[i]), last);
+    }
+    if (resid) *resid = last;
+    free(Qt);
+    return 0;
+}
+
+/* dtcalc (src/Parabolic3D/NS/NS/stac3m/dtcalc.f90:9-58): minimum over all
+ * elements of 1/dti, before the CFL factor.  It reads gmet(d,:) at the
+ * Lobatto index equal to the Gauss index, as the reference does. */
+double or_dtcalc(const or_mesh *mesh, const or_phys *ph, const double *JQ)
+{
+    ops_t o;
+    if (make_ops(mesh->p, &o)) return -1.0;
+    const int ncg = o.ncg, nc = o.nc, np = o.np, ns = o.ns;
+    const double gamma = ph->gamma;
+    double dt = DBL_MAX;
+    for (int e = 0; e < mesh->K; ++e) {
+        const double *q = JQ + (size_t)e * NEQ * np;
+        for (int l = 0; l < ncg; ++l) {
+            const double dcz = o.cx[l + 1] - o.cx[l];
+            for (int m = 0; m < ncg; ++m) {
+                const double dcy = o.cx[m + 1] - o.cx[m];
+                for (int n = 0; n < ncg; ++n) {
+                    const double dcx = o.cx[n + 1] - o.cx[n];
+                    const int pt = n + ncg * (m + ncg * l);
+                    const double u0 = fabs(q[1 * np + pt] / q[0 * np + pt]);
+                    const double v0 = fabs(q[2 * np + pt] / q[0 * np + pt]);
+                    const double w0 = fabs(q[3 * np + pt] / q[0 * np + pt]);
+                    const double p = (gamma - 1.0) * (q[4 * np + pt] - 0.5 * q[0 * np + pt] * (u0 * u0 + v0 * v0 + w0 * w0));
+                    const double a = sqrt(gamma * p / q[0 * np + pt]);
+                    const double *g1 = mesh->gmetx + ((size_t)e * ns + n + nc * (m + ncg * l)) * 3;
+                    const double *g2 = mesh->gmety + ((size_t)e * ns + n + ncg * (m + nc * l)) * 3;
+                    const double *g3 = mesh->gmetz + ((size_t)e * ns + n + ncg * (m + ncg * l)) * 3;
+                    const double uu = (u0 + a) * g1[0] + (v0 + a) * g1[1] + (w0 + a) * g1[2];
+                    const double vv = (u0 + a) * g2[0] + (v0 + a) * g2[1] + (w0 + a) * g2[2];
+                    const double ww = (u0 + a) * g3[0] + (v0 + a) * g3[1] + (w0 + a) * g3[2];
+                    const double jac = mesh->jacob[(size_t)e * np + pt];
+                    const double xlam = fabs(uu * jac), ylam = fabs(vv * jac), zlam = fabs(ww * jac);
+                    const double dti = xlam / dcx + ylam / dcy + zlam / dcz + 1.0e-14;
+                    dt = fmin(dt, 1.0 / dti);
+                }
+            }
+        }
+    }
+    return dt;
+}

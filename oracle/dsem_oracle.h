@@ -1,0 +1,75 @@
+/*
+ * dsem_oracle.h -- CPU oracle for the DSEM staggered-grid RHS and the
+ * 5-stage low-storage RK4 step.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is a plain-C FP64 restatement of the
+ * reference algorithm, used as the checker by tests/, by
+ * __graft_entry__.smoke() and by bench.py's cpu_baseline leg.  It is never
+ * linked into, loaded by, or called from the product library
+ * (dsem_amd/csrc) and the product path never falls back to it.
+ *
+ * Array layouts (shared with the HIP library so results compare directly):
+ *   ncg = p+1 Gauss points, nc = p+2 Lobatto points per direction.
+ *   Gauss point      pt(n,m,l)  = n + ncg*(m + ncg*l)               np = ncg^3
+ *   staggered grid x (Lobatto in xi):   j + nc*(m + ncg*l)            ns = nc*ncg^2
+ *   staggered grid y (Lobatto in eta):  n + ncg*(j + nc*l)
+ *   staggered grid z (Lobatto in zeta): n + ncg*(m + ncg*j)
+ *   mortar point     (i,j) = i + ncg*j, (i,j) = (xi,zeta) for faces 1,2,
+ *                    (xi,eta) for faces 3,5, (eta,zeta) for faces 4,6.
+ *   state            JQ[K][5][np]  (J-weighted conserved variables)
+ *   jacob            [K][np]          (= 1/J)
+ *   gmetg            [K][np][3][3]    gmetg[i][j] = d xi_i / d x_j
+ *   gmet{x,y,z}      [K][ns][3]       row d of J * grad xi on grid d
+ *   mortars          m_elem[M][2], m_face[M][2] (faces 1..6, side 0 = master),
+ *                    m_sign[M], m_nsign[M], nhat[M][nf][3], norm[M][nf]
+ */
+#ifndef DSEM_ORACLE_H
+#define DSEM_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct {
+    int32_t K, p, nmort, pad_;
+    const double *jacob, *gmetg, *gmetx, *gmety, *gmetz;
+    const int32_t *m_elem, *m_face, *m_sign, *m_nsign;
+    const double *nhat, *norm;
+} or_mesh;
+
+typedef struct {
+    double gamma, re, pr, mach;
+} or_phys;
+
+/* Optional intermediate outputs of or_rhs (any pointer may be NULL). */
+typedef struct {
+    double *stag;   /* [K][3][5][ns] staggered states after face averaging  */
+    double *flux;   /* [K][3][5][ns] f,g,h after all face/viscous updates   */
+    double *grad;   /* [K][3][5][np] Qx,Qy,Qz at Gauss points               */
+    double *visc;   /* [K][3][5][ns] r,s,t                                   */
+} or_debug;
+
+int  or_ops(int p, double *cx, double *cxg, double *dmx, double *bx);
+void or_rkcoefs(double *ark, double *brk, double *crk);
+int  or_geometry(int K, int p, const double *corners, double *xg, double *jacob,
+                 double *gmetg, double *gmetx, double *gmety, double *gmetz);
+int  or_mortar_geometry(int M, int p, const double *corners, const int32_t *m_elem,
+                        const int32_t *m_face, double *nhat, double *norm);
+
+void or_euler_flux(const or_phys *ph, const double *Q, double *f, double *g, double *h);
+void or_riemann(const or_phys *ph, const double *nhat, double ds, const double *Ql,
+                const double *Qr, double *flux);
+void or_vis_flux(const or_phys *ph, const double *Q, const double *Qx, const double *Qy,
+                 const double *Qz, double amu, double *f, double *g, double *h);
+
+int  or_rhs(const or_mesh *mesh, const or_phys *ph, const double *JQ, const double *muh,
+            double *Qt, or_debug *dbg, int nthreads);
+int  or_rk_step(const or_mesh *mesh, const or_phys *ph, double *JQ, double *G, double dt,
+                const double *muh, double *resid, int nthreads);
+double or_dtcalc(const or_mesh *mesh, const or_phys *ph, const double *JQ);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
