@@ -1,0 +1,177 @@
+"""Throughput of the DSEM RHS + RK4(5) step on one MI355X.
+
+    python bench.py [--gpus 1] [--steps 20] [--warmup 3]
+
+Workload: Taylor-Green vortex on the periodic cube, 22^3 hex8 elements at
+p = 8 (9^3 Gauss points per element, 38,811,960 DOF), FP64, state resident in
+HBM.  A timed step is one full Carpenter-Kennedy RK4(5) step: five RHS
+evaluations with their register updates.  The metric is GDOF/s per RHS
+evaluation = DOF * 5 * steps / time / 1e9.
+
+Prints one JSON line with the throughput, a roofline entry for the dominant
+kernel (per-kernel HIP-event durations on the library's stream) and a CPU
+baseline measured with the oracle on the host cores.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from dsem_amd import cases  # noqa: E402
+from dsem_amd import mesh as M  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+
+
+def kernel_work(n_elem: int, nmort: int, p: int) -> dict:
+    """Algorithmic bytes / flops per launch for every kernel (FP64).
+
+    Bytes count each array the kernel must read or write once (per-element
+    metrics on the affine box).  Flops count the tensor-product contractions
+    plus the pointwise flux arithmetic.
+    """
+    ncg, nc = p + 1, p + 2
+    npt, ns, nf = ncg ** 3, nc * ncg ** 2, ncg ** 2
+    K, Mm = n_elem, nmort
+    d = 8
+    w = {}
+    w["k_traces"] = dict(bytes=K * (5 * npt + 6 * 5 * nf) * d, flops=K * 6 * 5 * nf * 2 * ncg)
+    w["k_face_inv"] = dict(bytes=Mm * nf * (10 + 10) * d, flops=Mm * nf * 120)
+    w["k_grad"] = dict(bytes=K * (5 * npt + 6 * 5 * nf + 15 * npt) * d,
+                       flops=K * 5 * npt * 3 * 2 * (ncg + 2) + K * 15 * npt * 5)
+    # interpolation of Q to 3 grids, prolongation of 15 gradients to 3 grids,
+    # interior divergence, plus ~260 flops of pointwise Euler+viscous flux per point
+    w["k_volume"] = dict(bytes=K * (5 * npt + 15 * npt + 6 * 5 * nf + 6 * 5 * nf + 5 * npt) * d,
+                         flops=K * (3 * 5 * ns * 2 * ncg + 3 * 15 * ns * 2 * ncg + 3 * 5 * npt * 2 * (nc - 2)
+                                    + 3 * ns * 260))
+    w["k_face_visc"] = dict(bytes=Mm * nf * (10 + 5 + 5) * d, flops=Mm * nf * 5 * 6)
+    w["k_update"] = dict(bytes=K * npt * 5 * (1 + 2 + 2) * d + Mm * nf * 5 * d * 2, flops=K * npt * 5 * 16)
+    return w
+
+
+def cpu_baseline(mesh, JQ, args) -> dict:
+    """Oracle RHS on the host cores over a bounded sample of the workload."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
+    cores = max(1, min(cores, int(os.environ.get("OMP_NUM_THREADS", cores)), 16))
+    g = O.Geometry(mesh)
+    ph = O.phys(re=1600.0, pr=0.71)
+    O.rhs(g, JQ, ph, nthreads=cores)              # first touch
+    n_eval = 0
+    t0 = time.perf_counter()
+    while True:
+        O.rhs(g, JQ, ph, nthreads=cores)
+        n_eval += 1
+        if time.perf_counter() - t0 > args.cpu_seconds or n_eval >= 8:
+            break
+    dt = time.perf_counter() - t0
+    return {"value": round(mesh.dof * n_eval / dt / 1e9, 5), "unit": "GDOF/s per RHS", "cores": cores,
+            "kind": "port",
+            "sample": f"{n_eval} full RHS evaluations of the same 22^3 p=8 state by the FP64 C oracle "
+                      f"(OpenMP, {cores} threads), {dt:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--elements", type=int, default=22, help="elements per direction")
+    ap.add_argument("--p", type=int, default=8)
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32", "bf16x3"])
+    ap.add_argument("--metrics", default="per_point", choices=["auto", "per_point", "per_element"])
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-roofline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (multi-GPU sharding is a "
+                                   "design note only)"}))
+        sys.exit(2)
+
+    from dsem_amd.solver import DSEM
+    mesh = M.periodic_box(args.elements, args.p)
+    s = DSEM(mesh, re=1600.0, pr=0.71, mach=1.0, precision=args.precision, metrics=args.metrics)
+    xg, jac = s.gauss_points()
+    JQ = cases.tgv_state(xg, jac, "reference", perturb=1e-3, seed=1234)
+    s.set_state(JQ)
+    dt = s.dtcalc(cfl=0.5)
+
+    for _ in range(args.warmup):
+        s.steps(1, dt, check=False)
+    s.synchronize()
+    t0 = time.perf_counter()
+    s.steps(args.steps, dt, check=False)
+    s.synchronize()
+    elapsed = time.perf_counter() - t0
+    resid = s.tstep(0.0, dt)          # untimed: NaN/Inf check of the run
+    if not math.isfinite(resid):
+        raise RuntimeError("non-finite residual")
+
+    dof = mesh.dof
+    ms_step = elapsed * 1e3 / args.steps
+    value = dof * 5 * args.steps / elapsed / 1e9
+
+    out = {"metric": "GDOF/s per RHS (FP64 vs mixed @ rel L2<=1e-6); adjoint/forward time ratio",
+           "value": round(value, 4), "unit": "GDOF/s per RHS", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": round(ms_step, 4), "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f64" if args.precision == "fp64" else args.precision,
+           "data": "synthetic (TGV initial field with seeded 1e-3 relative perturbation, rng 1234)",
+           "config": {"workload": f"Taylor-Green vortex, {args.elements}^3 hex8 elements, p={args.p}, "
+                                  f"FP64 state, RK4(5) 2N step = 5 RHS", "elements": mesh.K, "p": args.p,
+                      "dof": dof, "mortars": mesh.nmortar, "precision": args.precision, "metrics": args.metrics,
+                      "gdof_per_s_per_rk_step": round(dof * args.steps / elapsed / 1e9, 4)}}
+
+    if not args.no_roofline:
+        s.timing(True)
+        s.steps(1, dt, check=False)
+        rep = s.timing_report()
+        s.timing(False)
+        work = kernel_work(mesh.K, mesh.nmortar, args.p)
+        per = {}
+        for name, (tot_ms, n) in rep.items():
+            if n == 0 or name not in work:
+                continue
+            avg = tot_ms / n
+            per[name] = {"avg_ms": round(avg, 5), "launches": n,
+                         "GB/s": round(work[name]["bytes"] / (avg * 1e-3) / 1e9, 1),
+                         "TFLOP/s": round(work[name]["flops"] / (avg * 1e-3) / 1e12, 2)}
+        dom = max(per, key=lambda k: per[k]["avg_ms"])
+        wk = work[dom]
+        avg_s = per[dom]["avg_ms"] * 1e-3
+        ai = wk["flops"] / wk["bytes"]
+        ridge = FP64_PEAK_TFLOPS * 1e12 / (HBM_PEAK_GBS * 1e9)
+        if ai >= ridge:
+            ach = wk["flops"] / avg_s / 1e12
+            out["roofline"] = {"bound": "mfma", "kernel": dom, "achieved": round(ach, 3), "peak": FP64_PEAK_TFLOPS,
+                               "unit": "TFLOP/s", "frac": round(ach / FP64_PEAK_TFLOPS, 4), "traffic": None,
+                               "note": "FP64 peak; algorithmic flops per launch = contraction + pointwise flux "
+                                       "flops (bench.kernel_work)"}
+        else:
+            ach = wk["bytes"] / avg_s / 1e9
+            out["roofline"] = {"bound": "hbm", "kernel": dom, "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
+                               "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None}
+        out["kernels"] = per
+
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(mesh, JQ, args)
+    s.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

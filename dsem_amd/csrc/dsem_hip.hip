@@ -1,0 +1,1043 @@
+// dsem_hip.hip -- MI355X (gfx950) kernels and C ABI for the DSEM
+// staggered-grid spectral-element RHS and the 5-stage RK4 2N step.
+//
+// Pipeline of one RHS evaluation (one workgroup per element for volume work,
+// one work-item per mortar point for face work; all FP64):
+//
+//   k_traces        J*Q -> Q, Gauss->Lobatto interpolation evaluated at the
+//                   six faces only (Int_fluxes' face values,
+//                   fluxesH_newa.f90:262-352; send_q_to_mortars, send_q.f90:491-764)
+//   k_face_inv      per mortar point: Roe flux with entropy fix scaled by the
+//                   face norm and the averaged state (interface_flux,
+//                   mortar_operations.f90:151-238; riemann, PhysLib.f90:231-320;
+//                   V_Avg_Soln, mortar_operations.f90:781-837)
+//   k_grad          Gauss-point gradients of Q from the staggered states whose
+//                   face values are the mortar averages (Gauss_Deriv,
+//                   SolnDerivsc.f90:14-83), using the combined operator
+//                   D_int*B for the interior Lobatto points
+//   k_volume        per staggered grid: interpolated state, prolonged
+//                   gradients, Euler + viscous contravariant fluxes at
+//                   interior Lobatto points, viscous face traces r/norm, and
+//                   the interior part of the flux divergence
+//                   (Int_fluxes fluxesH_newa.f90:274-352, Vis_Fluxes
+//                   Vis_Fluxes_new.f90:84-228, send_flux_to_mortars
+//                   send_flux.f90:243-457, update_interior updatea.f90:100-113)
+//   k_face_visc     per mortar point: averaged viscous flux subtracted from the
+//                   Roe flux (mortar_vis_flux ViscousBC_slide.f90:231-271,
+//                   send_flux_to_faces_add send_flux.f90:699-921)
+//   k_update        face part of the divergence, Qt, and the RK register
+//                   update g = a g + Qt, JQ += c dt g (updatea.f90:213-227)
+//
+// Element-major structure-of-arrays layouts (ncg = p+1, nc = p+2):
+//   JQ, G, Qtp [K][5][np]        point n + ncg*(m + ncg*l)
+//   Gr         [K][15][np]       (c*5 + v): dQ_v/dx_c at Gauss points
+//   T, FV      [K][6][5][nf]     face traces, face f-1, point i + ncg*j
+//   QA, FI, FT [M][5][nf]        per mortar (master face indexing)
+//   metrics    jacob [K][npm], gmetg [K][9][npm], gmet_d [K][3][nsm],
+//              nhat [M][3][nfm], norm [M][nfm]; npm = np (per point) or 1
+//              (per element, affine meshes) -- selected by the stride ms.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/dsem_hip.h"
+#include "geom.h"
+
+namespace {
+
+constexpr int NEQ = 5;
+constexpr int BLOCK = 256;
+
+// operators for every order p = 1..8, indexed [p]
+__constant__ double c_D[9][90];   // D[k*nc + j]
+__constant__ double c_B[9][90];   // B[j*ncg + k]
+__constant__ double c_M[9][81];   // M[k*ncg + k2]
+__constant__ double c_cx[9][10];  // Lobatto points
+
+template <int P>
+struct Sz {
+    static constexpr int NCG = P + 1, NC = P + 2;
+    static constexpr int NP = NCG * NCG * NCG, NS = NC * NCG * NCG, NF = NCG * NCG;
+    static constexpr int R = (NP + BLOCK - 1) / BLOCK;
+};
+
+struct DevMesh {
+    int K, M, ms;                 // ms: 1 per-point metrics, 0 per-element
+    const double *jacob, *gmetg, *gmet0, *gmet1, *gmet2;
+    const int *e2m;               // [K][6] (mortar << 1) | side
+    const int *m_elem, *m_face, *m_sign, *m_nsign;
+    const double *nhat, *norm;
+};
+
+struct DevPhys {
+    double gamma, re, pr, mach;
+};
+
+// Lobatto index j of grid d at line (a,b) -> staggered index
+template <int P>
+__device__ __forceinline__ int sidx(int d, int j, int a, int b)
+{
+    using S = Sz<P>;
+    return d == 0 ? j + S::NC * (a + S::NCG * b) : (d == 1 ? a + S::NCG * (j + S::NC * b) : a + S::NCG * (b + S::NCG * j));
+}
+
+// Gauss index k along direction d at line (a,b)
+template <int P>
+__device__ __forceinline__ int gidx(int d, int k, int a, int b)
+{
+    using S = Sz<P>;
+    return d == 0 ? k + S::NCG * (a + S::NCG * b) : (d == 1 ? a + S::NCG * (k + S::NCG * b) : a + S::NCG * (b + S::NCG * k));
+}
+
+// face (1..6) reached at Lobatto end j of grid d
+template <int P>
+__device__ __forceinline__ int face_of(int d, int j)
+{
+    const bool hi = (j != 0);
+    return d == 0 ? (hi ? 4 : 6) : (d == 1 ? (hi ? 2 : 1) : (hi ? 5 : 3));
+}
+
+// ------------------------------------------------------------ pointwise physics
+// Euler fluxes of PhysLib.f90:17-124, contracted with one metric row.
+__device__ __forceinline__ void euler_contravariant(const DevPhys &ph, const double *Q, double g0, double g1, double g2,
+                                                    double *out)
+{
+    const double rho = Q[0], rhou = Q[1], rhov = Q[2], rhow = Q[3], rhoe = Q[4];
+    const double u = rhou / rho, v = rhov / rho, w = rhow / rho;
+    const double p = (ph.gamma - 1.0) * (rhoe - 0.5 * rho * (u * u + v * v + w * w));
+    const double f[5] = {rhou, p + rhou * u, rhou * v, rhou * w, u * (rhoe + p)};
+    const double g[5] = {rhov, rhou * v, p + rhov * v, rhow * v, v * (rhoe + p)};
+    const double h[5] = {rhow, rhou * w, rhov * w, p + rhow * w, w * (rhoe + p)};
+#pragma unroll
+    for (int v2 = 0; v2 < 5; ++v2) out[v2] = g0 * f[v2] + g1 * g[v2] + g2 * h[v2];
+}
+
+// x/y/z_vis_flux of PhysLib.f90:419-583 contracted with one metric row.
+__device__ __forceinline__ void viscous_contravariant(const DevPhys &ph, const double *Q, const double *Qx,
+                                                      const double *Qy, const double *Qz, double amu, double g0,
+                                                      double g1, double g2, double *r)
+{
+    const double gamma = ph.gamma, mach = ph.mach;
+    const double rho = Q[0];
+    const double u = Q[1] / rho, v = Q[2] / rho, w = Q[3] / rho;
+    const double q2 = u * u + v * v + w * w;
+    const double p = (gamma - 1.0) * (Q[4] - 0.5 * rho * q2);
+    const double ux = (Qx[1] - u * Qx[0]) / rho, uy = (Qy[1] - u * Qy[0]) / rho, uz = (Qz[1] - u * Qz[0]) / rho;
+    const double vx = (Qx[2] - v * Qx[0]) / rho, vy = (Qy[2] - v * Qy[0]) / rho, vz = (Qz[2] - v * Qz[0]) / rho;
+    const double wx = (Qx[3] - w * Qx[0]) / rho, wy = (Qy[3] - w * Qy[0]) / rho, wz = (Qz[3] - w * Qz[0]) / rho;
+    const double akappa = 1.0 + (amu - 1.0) * 2.0e-1 / (gamma - 1.0);
+    const double div = ux + vy + wz;
+    const double tfac = mach * mach * gamma;
+    const double hfac = akappa / (ph.pr * mach * mach * (gamma - 1.0));
+    const double px = (gamma - 1.0) * (Qx[4] - 0.5 * Qx[0] * q2 - rho * (u * ux + v * vx + w * wx));
+    const double py = (gamma - 1.0) * (Qy[4] - 0.5 * Qy[0] * q2 - rho * (u * uy + v * vy + w * wy));
+    const double pz = (gamma - 1.0) * (Qz[4] - 0.5 * Qz[0] * q2 - rho * (u * uz + v * vz + w * wz));
+    const double tx = (px - p * Qx[0] / rho) * tfac / rho;
+    const double ty = (py - p * Qy[0] / rho) * tfac / rho;
+    const double tz = (pz - p * Qz[0] / rho) * tfac / rho;
+    const double txx = 2.0 * amu * (ux - div / 3.0), tyy = 2.0 * amu * (vy - div / 3.0), tzz = 2.0 * amu * (wz - div / 3.0);
+    const double txy = amu * (uy + vx), txz = amu * (wx + uz), tyz = amu * (wy + vz);
+    const double f4 = u * txx + v * txy + w * txz + hfac * tx;
+    const double g4 = v * tyy + u * txy + w * tyz + hfac * ty;
+    const double h4 = v * tyz + u * txz + w * tzz + hfac * tz;
+    r[0] = 0.0;
+    r[1] = g0 * txx + g1 * txy + g2 * txz;
+    r[2] = g0 * txy + g1 * tyy + g2 * tyz;
+    r[3] = g0 * txz + g1 * tyz + g2 * tzz;
+    r[4] = g0 * f4 + g1 * g4 + g2 * h4;
+}
+
+// Roe flux with entropy fix (PhysLib.f90:231-320)
+__device__ __forceinline__ void roe_flux(const DevPhys &ph, double n0, double n1, double n2, double ds, const double *Ql,
+                                         const double *Qr, double *flux)
+{
+    const double gamma = ph.gamma;
+    const double rho = Ql[0], rhon = Qr[0];
+    const double ul = Ql[1] / rho, vl = Ql[2] / rho, wl = Ql[3] / rho;
+    const double pleft = (gamma - 1.0) * (Ql[4] - 0.5 / rho * (Ql[1] * Ql[1] + Ql[2] * Ql[2] + Ql[3] * Ql[3]));
+    const double ur = Qr[1] / rhon, vr = Qr[2] / rhon, wr = Qr[3] / rhon;
+    const double pright = (gamma - 1.0) * (Qr[4] - 0.5 / rhon * (Qr[1] * Qr[1] + Qr[2] * Qr[2] + Qr[3] * Qr[3]));
+    const double ql = n0 * ul + n1 * vl + n2 * wl;
+    const double qr = n0 * ur + n1 * vr + n2 * wr;
+    const double gg = gamma / (gamma - 1.0);
+    const double hl = 0.5 * (ul * ul + vl * vl + wl * wl) + gg * pleft / rho;
+    const double hr = 0.5 * (ur * ur + vr * vr + wr * wr) + gg * pright / rhon;
+    const double rtd = sqrt(rho * rhon);
+    const double betal = rho / (rho + rtd);
+    const double betar = 1.0 - betal;
+    const double utd = betal * ul + betar * ur, vtd = betal * vl + betar * vr, wtd = betal * wl + betar * wr;
+    const double htd = betal * hl + betar * hr;
+    const double atd2 = (gamma - 1.0) * (htd - 0.5 * (utd * utd + vtd * vtd + wtd * wtd));
+    const double atd = sqrt(atd2);
+    const double qtd = utd * n0 + vtd * n1 + wtd * n2;
+    const double hcoef = (gamma + 1.0) / 4.0 * atd / rtd;
+    if (qtd >= 0.0) {
+        const double dw1 = 0.5 * ((pright - pleft) / atd2 - (qr - ql) * rtd / atd);
+        const double sp1 = qtd - atd;
+        const double sp1m = fmin(sp1, 0.0);
+        const double eta1 = fmax(-fabs(sp1) - hcoef * dw1, 0.0);
+        const double udw1 = dw1 * (sp1m - 0.5 * eta1);
+        const double rql = rho * ql;
+        flux[0] = ds * (rql + udw1);
+        flux[1] = ds * (rql * ul + pleft * n0 + udw1 * (utd - atd * n0));
+        flux[2] = ds * (rql * vl + pleft * n1 + udw1 * (vtd - atd * n1));
+        flux[3] = ds * (rql * wl + pleft * n2 + udw1 * (wtd - atd * n2));
+        flux[4] = ds * (rql * hl + udw1 * (htd - qtd * atd));
+    } else {
+        const double dw4 = 0.5 * ((pright - pleft) / atd2 + (qr - ql) * rtd / atd);
+        const double sp4 = qtd + atd;
+        const double sp4p = fmax(sp4, 0.0);
+        const double eta4 = fmax(-fabs(sp4) + hcoef * dw4, 0.0);
+        const double udw4 = dw4 * (sp4p + 0.5 * eta4);
+        const double rqr = rhon * qr;
+        flux[0] = ds * (rqr - udw4);
+        flux[1] = ds * (rqr * ur + pright * n0 - udw4 * (utd + atd * n0));
+        flux[2] = ds * (rqr * vr + pright * n1 - udw4 * (vtd + atd * n1));
+        flux[3] = ds * (rqr * wr + pright * n2 - udw4 * (wtd + atd * n2));
+        flux[4] = ds * (rqr * hr - udw4 * (htd + qtd * atd));
+    }
+}
+
+// ------------------------------------------------------------ kernels
+template <int P>
+__global__ __launch_bounds__(BLOCK) void k_traces(const double *__restrict__ JQ, DevMesh m, double *__restrict__ T)
+{
+    using S = Sz<P>;
+    const int e = blockIdx.x;
+    const double *q = JQ + (size_t)e * NEQ * S::NP;
+    const double *jac = m.jacob + (size_t)e * (m.ms ? S::NP : 1);
+    double *t = T + (size_t)e * 6 * NEQ * S::NF;
+    for (int i = threadIdx.x; i < 6 * NEQ * S::NF; i += BLOCK) {
+        const int qf = i % S::NF, v = (i / S::NF) % NEQ, f = i / (NEQ * S::NF) + 1;
+        const int a = qf % S::NCG, b = qf / S::NCG;
+        const int d = (f == 4 || f == 6) ? 0 : ((f == 1 || f == 2) ? 1 : 2);
+        const int j = (f == 2 || f == 4 || f == 5) ? S::NC - 1 : 0;
+        double s = 0.0;
+#pragma unroll
+        for (int k = 0; k < S::NCG; ++k) {
+            const int pt = gidx<P>(d, k, a, b);
+            s = fma(c_B[P][j * S::NCG + k], q[v * S::NP + pt] * jac[pt * m.ms], s);
+        }
+        t[i] = s;
+    }
+}
+
+template <int P>
+__global__ __launch_bounds__(BLOCK) void k_face_inv(const double *__restrict__ T, DevMesh m, DevPhys ph,
+                                                     double *__restrict__ FI, double *__restrict__ QA)
+{
+    using S = Sz<P>;
+    const int idx = blockIdx.x * BLOCK + threadIdx.x;
+    if (idx >= m.M * S::NF) return;
+    const int mm = idx / S::NF, q = idx % S::NF;
+    const int em = m.m_elem[2 * mm], es = m.m_elem[2 * mm + 1];
+    const int fm = m.m_face[2 * mm], fs = m.m_face[2 * mm + 1];
+    double Qm[5], Qs[5], fl[5];
+#pragma unroll
+    for (int v = 0; v < 5; ++v) {
+        Qm[v] = T[(((size_t)em * 6 + fm - 1) * NEQ + v) * S::NF + q];
+        Qs[v] = T[(((size_t)es * 6 + fs - 1) * NEQ + v) * S::NF + q];
+    }
+    const int nfm = m.ms ? S::NF : 1, qq = q * m.ms;
+    const double n0 = m.nhat[((size_t)mm * 3 + 0) * nfm + qq];
+    const double n1 = m.nhat[((size_t)mm * 3 + 1) * nfm + qq];
+    const double n2 = m.nhat[((size_t)mm * 3 + 2) * nfm + qq];
+    const double ds = m.norm[(size_t)mm * nfm + qq];
+    if (fm == 1 || fm == 3 || fm == 6) roe_flux(ph, n0, n1, n2, ds, Qs, Qm, fl);
+    else roe_flux(ph, n0, n1, n2, ds, Qm, Qs, fl);
+#pragma unroll
+    for (int v = 0; v < 5; ++v) {
+        FI[((size_t)mm * NEQ + v) * S::NF + q] = fl[v];
+        QA[((size_t)mm * NEQ + v) * S::NF + q] = (Qm[v] + Qs[v]) * 0.5;
+    }
+}
+
+template <int P>
+__device__ __forceinline__ void load_physical_state(const double *__restrict__ JQ, const DevMesh &m, int e, double *sQ)
+{
+    using S = Sz<P>;
+    const double *q = JQ + (size_t)e * NEQ * S::NP;
+    const double *jac = m.jacob + (size_t)e * (m.ms ? S::NP : 1);
+    for (int i = threadIdx.x; i < NEQ * S::NP; i += BLOCK) sQ[i] = q[i] * jac[(i % S::NP) * m.ms];
+}
+
+template <int P>
+__global__ __launch_bounds__(BLOCK) void k_grad(const double *__restrict__ JQ, DevMesh m, const double *__restrict__ QA,
+                                                 double *__restrict__ Gr)
+{
+    using S = Sz<P>;
+    __shared__ double sQ[NEQ * S::NP];
+    const int e = blockIdx.x;
+    load_physical_state<P>(JQ, m, e, sQ);
+    __syncthreads();
+    const int *e2m = m.e2m + (size_t)e * 6;
+    const int npm = m.ms ? S::NP : 1;
+    const double *gg = m.gmetg + (size_t)e * 9 * npm;
+    for (int i = threadIdx.x; i < NEQ * S::NP; i += BLOCK) {
+        const int v = i / S::NP, pt = i % S::NP;
+        const int n = pt % S::NCG, mi = (pt / S::NCG) % S::NCG, l = pt / (S::NCG * S::NCG);
+        const double *sv = sQ + v * S::NP;
+        double w[3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            const int k = d == 0 ? n : (d == 1 ? mi : l);
+            const int a = d == 0 ? mi : n;
+            const int b = d == 2 ? mi : l;
+            double s = 0.0;
+#pragma unroll
+            for (int k2 = 0; k2 < S::NCG; ++k2) s = fma(c_M[P][k * S::NCG + k2], sv[gidx<P>(d, k2, a, b)], s);
+            const int f0 = face_of<P>(d, 0), f1 = face_of<P>(d, 1);
+            const int q = a + S::NCG * b;
+            const double qa0 = QA[((size_t)(e2m[f0 - 1] >> 1) * NEQ + v) * S::NF + q];
+            const double qa1 = QA[((size_t)(e2m[f1 - 1] >> 1) * NEQ + v) * S::NF + q];
+            s = fma(c_D[P][k * S::NC + 0], qa0, s);
+            s = fma(c_D[P][k * S::NC + S::NC - 1], qa1, s);
+            w[d] = s;
+        }
+        const int pq = pt * m.ms;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const double g0 = gg[(0 * 3 + c) * npm + pq], g1 = gg[(1 * 3 + c) * npm + pq], g2 = gg[(2 * 3 + c) * npm + pq];
+            Gr[((size_t)e * 15 + c * 5 + v) * S::NP + pt] = g0 * w[0] + g1 * w[1] + g2 * w[2];
+        }
+    }
+}
+
+template <int P>
+__global__ __launch_bounds__(BLOCK) void k_volume(const double *__restrict__ JQ, DevMesh m, DevPhys ph,
+                                                   const double *__restrict__ QA, const double *__restrict__ Gr,
+                                                   const double *__restrict__ muh, double *__restrict__ FV,
+                                                   double *__restrict__ Qtp)
+{
+    using S = Sz<P>;
+    __shared__ double sQ[NEQ * S::NP];
+    __shared__ double sF[NEQ * S::NS];
+    const int e = blockIdx.x;
+    load_physical_state<P>(JQ, m, e, sQ);
+    const int *e2m = m.e2m + (size_t)e * 6;
+    const int nsm = m.ms ? S::NS : 1;
+    const int nfm = m.ms ? S::NF : 1;
+    const double *gre = Gr + (size_t)e * 15 * S::NP;
+    const double inv_re = 1.0 / ph.re;
+    double acc[S::R][5];
+#pragma unroll
+    for (int r = 0; r < S::R; ++r)
+#pragma unroll
+        for (int v = 0; v < 5; ++v) acc[r][v] = 0.0;
+
+    for (int d = 0; d < 3; ++d) {
+        const double *gmd = d == 0 ? m.gmet0 : (d == 1 ? m.gmet1 : m.gmet2);
+        gmd += (size_t)e * 3 * nsm;
+        __syncthreads();
+        for (int s = threadIdx.x; s < S::NS; s += BLOCK) {
+            int j, a, b;
+            if (d == 0) { j = s % S::NC; a = (s / S::NC) % S::NCG; b = s / (S::NC * S::NCG); }
+            else if (d == 1) { a = s % S::NCG; j = (s / S::NCG) % S::NC; b = s / (S::NCG * S::NC); }
+            else { a = s % S::NCG; b = (s / S::NCG) % S::NCG; j = s / (S::NCG * S::NCG); }
+            const bool face = (j == 0 || j == S::NC - 1);
+            int mort = 0, fc = 0;
+            const int q = a + S::NCG * b;
+            double Qv[5];
+            if (face) {
+                fc = face_of<P>(d, j);
+                mort = e2m[fc - 1] >> 1;
+#pragma unroll
+                for (int v = 0; v < 5; ++v) Qv[v] = QA[((size_t)mort * NEQ + v) * S::NF + q];
+            } else {
+#pragma unroll
+                for (int v = 0; v < 5; ++v) {
+                    double t = 0.0;
+#pragma unroll
+                    for (int k = 0; k < S::NCG; ++k) t = fma(c_B[P][j * S::NCG + k], sQ[v * S::NP + gidx<P>(d, k, a, b)], t);
+                    Qv[v] = t;
+                }
+            }
+            double Gx[3][5];
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+#pragma unroll
+                for (int v = 0; v < 5; ++v) {
+                    const double *gl = gre + (c * 5 + v) * S::NP;
+                    double t = 0.0;
+#pragma unroll
+                    for (int k = 0; k < S::NCG; ++k) t = fma(c_B[P][j * S::NCG + k], gl[gidx<P>(d, k, a, b)], t);
+                    Gx[c][v] = t;
+                }
+            double amu = 1.0;
+            if (muh) amu = amu + muh[((size_t)e * 3 + d) * S::NS + s];
+            const int sq = s * m.ms;
+            const double g0 = gmd[0 * nsm + sq], g1 = gmd[1 * nsm + sq], g2 = gmd[2 * nsm + sq];
+            double rv[5];
+            viscous_contravariant(ph, Qv, Gx[0], Gx[1], Gx[2], amu, g0, g1, g2, rv);
+            if (face) {
+                const double nrm = m.norm[(size_t)mort * nfm + q * m.ms];
+#pragma unroll
+                for (int v = 0; v < 5; ++v) FV[(((size_t)e * 6 + fc - 1) * NEQ + v) * S::NF + q] = rv[v] / nrm;
+            } else {
+                double fi[5];
+                euler_contravariant(ph, Qv, g0, g1, g2, fi);
+#pragma unroll
+                for (int v = 0; v < 5; ++v) sF[v * S::NS + s] = fi[v] - rv[v] * inv_re;
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < S::R; ++r) {
+            const int pt = threadIdx.x + r * BLOCK;
+            if (pt < S::NP) {
+                const int n = pt % S::NCG, mi = (pt / S::NCG) % S::NCG, l = pt / (S::NCG * S::NCG);
+                const int k = d == 0 ? n : (d == 1 ? mi : l);
+                const int a = d == 0 ? mi : n;
+                const int b = d == 2 ? mi : l;
+#pragma unroll
+                for (int v = 0; v < 5; ++v) {
+                    double t = 0.0;
+#pragma unroll
+                    for (int j = 1; j < S::NC - 1; ++j) t = fma(c_D[P][k * S::NC + j], sF[v * S::NS + sidx<P>(d, j, a, b)], t);
+                    acc[r][v] -= t;
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < S::R; ++r) {
+        const int pt = threadIdx.x + r * BLOCK;
+        if (pt < S::NP)
+#pragma unroll
+            for (int v = 0; v < 5; ++v) Qtp[((size_t)e * NEQ + v) * S::NP + pt] = acc[r][v];
+    }
+}
+
+template <int P>
+__global__ __launch_bounds__(BLOCK) void k_face_visc(const double *__restrict__ FV, const double *__restrict__ FI,
+                                                      DevMesh m, DevPhys ph, double *__restrict__ FT)
+{
+    using S = Sz<P>;
+    const int idx = blockIdx.x * BLOCK + threadIdx.x;
+    if (idx >= m.M * S::NF) return;
+    const int mm = idx / S::NF, q = idx % S::NF;
+    const int em = m.m_elem[2 * mm], es = m.m_elem[2 * mm + 1];
+    const int fm = m.m_face[2 * mm], fs = m.m_face[2 * mm + 1];
+    const int nfm = m.ms ? S::NF : 1;
+    const double nrm = m.norm[(size_t)mm * nfm + q * m.ms];
+    const bool flip = m.m_nsign[mm] == -1;
+#pragma unroll
+    for (int v = 0; v < 5; ++v) {
+        const double fvm = FV[(((size_t)em * 6 + fm - 1) * NEQ + v) * S::NF + q];
+        double fvs = FV[(((size_t)es * 6 + fs - 1) * NEQ + v) * S::NF + q];
+        if (flip) fvs = -fvs;
+        const double qb = 0.5 * (fvm + fvs) * nrm;
+        FT[((size_t)mm * NEQ + v) * S::NF + q] = FI[((size_t)mm * NEQ + v) * S::NF + q] - qb / ph.re;
+    }
+}
+
+struct StageArgs {
+    double a, cdt;                 // RK coefficients for this stage (a unused when first)
+    int first;                     // first stage: g = Qt
+    int rk;                        // 0: write Qt only, 1: RK update
+    unsigned long long *resid;     // max |Qt| of the last element (nullable)
+    unsigned int *nonfinite;       // set when any Qt is NaN/Inf (nullable)
+};
+
+template <int P>
+__global__ __launch_bounds__(BLOCK) void k_update(const double *__restrict__ Qtp, const double *__restrict__ FT, DevMesh m,
+                                                   StageArgs st, double *__restrict__ JQ, double *__restrict__ G,
+                                                   double *__restrict__ Qt)
+{
+    using S = Sz<P>;
+    const int e = blockIdx.x;
+    const int *e2m = m.e2m + (size_t)e * 6;
+    double mx = 0.0;
+    bool bad = false;
+    for (int i = threadIdx.x; i < NEQ * S::NP; i += BLOCK) {
+        const int v = i / S::NP, pt = i % S::NP;
+        const int n = pt % S::NCG, mi = (pt / S::NCG) % S::NCG, l = pt / (S::NCG * S::NCG);
+        double qt = Qtp[(size_t)e * NEQ * S::NP + i];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            const int k = d == 0 ? n : (d == 1 ? mi : l);
+            const int a = d == 0 ? mi : n;
+            const int b = d == 2 ? mi : l;
+            const int q = a + S::NCG * b;
+            const int f0 = face_of<P>(d, 0), f1 = face_of<P>(d, 1);
+            const int c0 = e2m[f0 - 1], c1 = e2m[f1 - 1];
+            double F0 = FT[((size_t)(c0 >> 1) * NEQ + v) * S::NF + q];
+            double F1 = FT[((size_t)(c1 >> 1) * NEQ + v) * S::NF + q];
+            if ((c0 & 1) && m.m_sign[c0 >> 1] == -1) F0 = -F0;
+            if ((c1 & 1) && m.m_sign[c1 >> 1] == -1) F1 = -F1;
+            qt = qt - fma(c_D[P][k * S::NC + 0], F0, c_D[P][k * S::NC + S::NC - 1] * F1);
+        }
+        const size_t gi = (size_t)e * NEQ * S::NP + i;
+        if (st.rk) {
+            const double g = st.first ? qt : fma(st.a, G[gi], qt);
+            G[gi] = g;
+            JQ[gi] = fma(st.cdt, g, JQ[gi]);
+        } else {
+            Qt[gi] = qt;
+        }
+        mx = fmax(mx, fabs(qt));
+        bad = bad || !isfinite(qt);
+    }
+    if (bad && st.nonfinite) atomicOr(st.nonfinite, 1u);
+    if (st.resid && e == m.K - 1) atomicMax(st.resid, (unsigned long long)__double_as_longlong(mx));
+}
+
+// dtcalc (dtcalc.f90:9-58): per point 1/dti, reduced to the minimum
+template <int P>
+__global__ __launch_bounds__(BLOCK) void k_dtcalc(const double *__restrict__ JQ, DevMesh m, DevPhys ph,
+                                                   unsigned long long *out)
+{
+    using S = Sz<P>;
+    const int e = blockIdx.x;
+    const double *q = JQ + (size_t)e * NEQ * S::NP;
+    const int nsm = m.ms ? S::NS : 1;
+    double best = 1.0e308;
+    for (int pt = threadIdx.x; pt < S::NP; pt += BLOCK) {
+        const int n = pt % S::NCG, mi = (pt / S::NCG) % S::NCG, l = pt / (S::NCG * S::NCG);
+        const double dcx = c_cx[P][n + 1] - c_cx[P][n];
+        const double dcy = c_cx[P][mi + 1] - c_cx[P][mi];
+        const double dcz = c_cx[P][l + 1] - c_cx[P][l];
+        const double r = q[0 * S::NP + pt];
+        const double u0 = fabs(q[1 * S::NP + pt] / r), v0 = fabs(q[2 * S::NP + pt] / r), w0 = fabs(q[3 * S::NP + pt] / r);
+        const double p = (ph.gamma - 1.0) * (q[4 * S::NP + pt] - 0.5 * r * (u0 * u0 + v0 * v0 + w0 * w0));
+        const double a = sqrt(ph.gamma * p / r);
+        const int s0 = (n + S::NC * (mi + S::NCG * l)) * m.ms;
+        const int s1 = (n + S::NCG * (mi + S::NC * l)) * m.ms;
+        const int s2 = (n + S::NCG * (mi + S::NCG * l)) * m.ms;
+        const double *g0 = m.gmet0 + (size_t)e * 3 * nsm, *g1 = m.gmet1 + (size_t)e * 3 * nsm, *g2 = m.gmet2 + (size_t)e * 3 * nsm;
+        const double uu = (u0 + a) * g0[s0] + (v0 + a) * g0[nsm + s0] + (w0 + a) * g0[2 * nsm + s0];
+        const double vv = (u0 + a) * g1[s1] + (v0 + a) * g1[nsm + s1] + (w0 + a) * g1[2 * nsm + s1];
+        const double ww = (u0 + a) * g2[s2] + (v0 + a) * g2[nsm + s2] + (w0 + a) * g2[2 * nsm + s2];
+        const double jac = m.jacob[(size_t)e * (m.ms ? S::NP : 1) + pt * m.ms];
+        const double dti = fabs(uu * jac) / dcx + fabs(vv * jac) / dcy + fabs(ww * jac) / dcz + 1.0e-14;
+        best = fmin(best, 1.0 / dti);
+    }
+    atomicMin(out, (unsigned long long)__double_as_longlong(best));
+}
+
+}  // namespace
+
+// ====================================================================== host
+namespace {
+
+enum KernelId { KID_TRACES = 0, KID_FACE_INV, KID_GRAD, KID_VOLUME, KID_FACE_VISC, KID_UPDATE, KID_DTCALC, KID_COUNT };
+const char *const kKernelNames[KID_COUNT] = {"k_traces", "k_face_inv", "k_grad", "k_volume",
+                                             "k_face_visc", "k_update", "k_dtcalc"};
+
+template <typename T>
+struct DevBuf {
+    T *ptr = nullptr;
+    size_t n = 0;
+};
+
+}  // namespace
+
+struct dsem_ctx {
+    int device = 0;
+    int K = 0, p = 0, M = 0, ncg = 0, nc = 0, np = 0, ns = 0, nf = 0;
+    int ms = 1;
+    dsem_params params{};
+    std::string err;
+    hipStream_t stream = nullptr;
+    dsem::Ops ops;
+    std::vector<double> h_xg, h_jacob;   // Gauss points and 1/J on the host
+    // device buffers
+    double *JQ = nullptr, *G = nullptr, *Qtp = nullptr, *Qt = nullptr;
+    double *Gr = nullptr, *T = nullptr, *FV = nullptr, *QA = nullptr, *FI = nullptr, *FT = nullptr;
+    double *jacob = nullptr, *gmetg = nullptr, *gmet[3] = {nullptr, nullptr, nullptr};
+    double *nhat = nullptr, *norm = nullptr, *muh = nullptr;
+    int *e2m = nullptr, *m_elem = nullptr, *m_face = nullptr, *m_sign = nullptr, *m_nsign = nullptr;
+    unsigned long long *d_scalars = nullptr;   // [0] resid bits, [1] dt bits, [2] nonfinite flag
+    std::vector<void *> allocations;
+    // timing
+    bool timing = false;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double t_ms[KID_COUNT] = {};
+    int64_t t_n[KID_COUNT] = {};
+};
+
+namespace {
+
+int set_err(dsem_ctx *c, int code, const std::string &msg)
+{
+    if (c) c->err = msg;
+    return code;
+}
+
+#define HIP_TRY(ctx, call)                                                                                  \
+    do {                                                                                                    \
+        hipError_t e_ = (call);                                                                             \
+        if (e_ != hipSuccess)                                                                               \
+            return set_err((ctx), DSEM_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_));         \
+    } while (0)
+
+template <typename T>
+int dev_alloc(dsem_ctx *c, T **p, size_t n)
+{
+    void *ptr = nullptr;
+    hipError_t e = hipMalloc(&ptr, sizeof(T) * (n ? n : 1));
+    if (e != hipSuccess) return set_err(c, DSEM_ERR_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    c->allocations.push_back(ptr);
+    *p = static_cast<T *>(ptr);
+    return DSEM_OK;
+}
+
+template <typename T>
+int upload(dsem_ctx *c, T **p, const std::vector<T> &h)
+{
+    int rc = dev_alloc(c, p, h.size());
+    if (rc) return rc;
+    HIP_TRY(c, hipMemcpy(*p, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice));
+    return DSEM_OK;
+}
+
+DevMesh dev_mesh(const dsem_ctx *c)
+{
+    DevMesh m;
+    m.K = c->K; m.M = c->M; m.ms = c->ms;
+    m.jacob = c->jacob; m.gmetg = c->gmetg; m.gmet0 = c->gmet[0]; m.gmet1 = c->gmet[1]; m.gmet2 = c->gmet[2];
+    m.e2m = c->e2m; m.m_elem = c->m_elem; m.m_face = c->m_face; m.m_sign = c->m_sign; m.m_nsign = c->m_nsign;
+    m.nhat = c->nhat; m.norm = c->norm;
+    return m;
+}
+
+DevPhys dev_phys(const dsem_ctx *c)
+{
+    return DevPhys{c->params.gamma, c->params.re, c->params.pr, c->params.mach};
+}
+
+// Launch helper: brackets the launch with events when timing is on.
+template <typename F>
+int launch(dsem_ctx *c, int kid, F &&fn)
+{
+    if (c->timing) HIP_TRY(c, hipEventRecord(c->ev0, c->stream));
+    fn();
+    HIP_TRY(c, hipGetLastError());
+    if (c->timing) {
+        HIP_TRY(c, hipEventRecord(c->ev1, c->stream));
+        HIP_TRY(c, hipEventSynchronize(c->ev1));
+        float ms = 0.f;
+        HIP_TRY(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+        c->t_ms[kid] += ms;
+        c->t_n[kid] += 1;
+    }
+    return DSEM_OK;
+}
+
+#define TRY(x)                 \
+    do {                       \
+        int rc_ = (x);         \
+        if (rc_) return rc_;   \
+    } while (0)
+
+// One RHS evaluation on state JQ; the final kernel either writes Qt or
+// performs an RK register update (st.rk).
+template <int P>
+int run_rhs(dsem_ctx *c, double *JQ, const StageArgs &st, double *Qt_out)
+{
+    using S = Sz<P>;
+    const DevMesh m = dev_mesh(c);
+    const DevPhys ph = dev_phys(c);
+    const int K = c->K;
+    const int face_blocks = (c->M * S::NF + BLOCK - 1) / BLOCK;
+    const double *muh = c->params.shock ? c->muh : nullptr;
+    hipStream_t s = c->stream;
+    TRY(launch(c, KID_TRACES, [&] { k_traces<P><<<K, BLOCK, 0, s>>>(JQ, m, c->T); }));
+    TRY(launch(c, KID_FACE_INV, [&] { k_face_inv<P><<<face_blocks, BLOCK, 0, s>>>(c->T, m, ph, c->FI, c->QA); }));
+    TRY(launch(c, KID_GRAD, [&] { k_grad<P><<<K, BLOCK, 0, s>>>(JQ, m, c->QA, c->Gr); }));
+    TRY(launch(c, KID_VOLUME,
+               [&] { k_volume<P><<<K, BLOCK, 0, s>>>(JQ, m, ph, c->QA, c->Gr, muh, c->FV, c->Qtp); }));
+    TRY(launch(c, KID_FACE_VISC, [&] { k_face_visc<P><<<face_blocks, BLOCK, 0, s>>>(c->FV, c->FI, m, ph, c->FT); }));
+    TRY(launch(c, KID_UPDATE, [&] { k_update<P><<<K, BLOCK, 0, s>>>(c->Qtp, c->FT, m, st, JQ, c->G, Qt_out); }));
+    return DSEM_OK;
+}
+
+template <int P>
+int run_dtcalc(dsem_ctx *c, unsigned long long *out)
+{
+    const DevMesh m = dev_mesh(c);
+    const DevPhys ph = dev_phys(c);
+    return launch(c, KID_DTCALC, [&] { k_dtcalc<P><<<c->K, BLOCK, 0, c->stream>>>(c->JQ, m, ph, out); });
+}
+
+#define DISPATCH_P(ctx, fn, ...)                        \
+    switch ((ctx)->p) {                                 \
+    case 1: return fn<1>(ctx, __VA_ARGS__);             \
+    case 2: return fn<2>(ctx, __VA_ARGS__);             \
+    case 3: return fn<3>(ctx, __VA_ARGS__);             \
+    case 4: return fn<4>(ctx, __VA_ARGS__);             \
+    case 5: return fn<5>(ctx, __VA_ARGS__);             \
+    case 6: return fn<6>(ctx, __VA_ARGS__);             \
+    case 7: return fn<7>(ctx, __VA_ARGS__);             \
+    case 8: return fn<8>(ctx, __VA_ARGS__);             \
+    default: return set_err(ctx, DSEM_ERR_ARG, "p");    \
+    }
+
+int rhs_dispatch(dsem_ctx *c, double *JQ, const StageArgs &st, double *Qt_out)
+{
+    DISPATCH_P(c, run_rhs, JQ, st, Qt_out);
+}
+
+int dtcalc_dispatch(dsem_ctx *c, unsigned long long *out)
+{
+    DISPATCH_P(c, run_dtcalc, out);
+}
+
+int upload_operators(dsem_ctx *c)
+{
+    double D[9][90] = {}, B[9][90] = {}, Mo[9][81] = {}, cx[9][10] = {};
+    for (int p = 1; p <= 8; ++p) {
+        dsem::Ops o;
+        dsem::make_ops(p, o);
+        std::memcpy(D[p], o.D, sizeof(o.D));
+        std::memcpy(B[p], o.B, sizeof(o.B));
+        std::memcpy(Mo[p], o.M, sizeof(o.M));
+        std::memcpy(cx[p], o.cx, sizeof(o.cx));
+    }
+    HIP_TRY(c, hipMemcpyToSymbol(HIP_SYMBOL(c_D), D, sizeof(D)));
+    HIP_TRY(c, hipMemcpyToSymbol(HIP_SYMBOL(c_B), B, sizeof(B)));
+    HIP_TRY(c, hipMemcpyToSymbol(HIP_SYMBOL(c_M), Mo, sizeof(Mo)));
+    HIP_TRY(c, hipMemcpyToSymbol(HIP_SYMBOL(c_cx), cx, sizeof(cx)));
+    return DSEM_OK;
+}
+
+int create_impl(const dsem_mesh_desc *mesh, const dsem_params *prm, int device, dsem_ctx *c)
+{
+    if (!mesh || !prm) return set_err(c, DSEM_ERR_ARG, "null mesh or params");
+    if (mesh->p < 1 || mesh->p > 8) return set_err(c, DSEM_ERR_ARG, "p must be in 1..8 (nc <= 10 as in size.f90)");
+    if (mesh->K < 1 || mesh->nmortar < 1 || !mesh->mortar_elem || !mesh->mortar_face || !mesh->corners)
+        return set_err(c, DSEM_ERR_ARG, "empty mesh");
+    if (prm->precision != DSEM_PREC_FP64)
+        return set_err(c, DSEM_ERR_UNSUPPORTED, "only the FP64 contraction path is built");
+    if (!(prm->gamma > 1.0) || !(prm->re > 0.0) || !(prm->pr > 0.0) || !(prm->mach > 0.0))
+        return set_err(c, DSEM_ERR_ARG, "physics parameters must be positive (gamma > 1)");
+    c->device = device;
+    c->K = mesh->K; c->p = mesh->p; c->M = mesh->nmortar;
+    c->ncg = c->p + 1; c->nc = c->p + 2;
+    c->np = c->ncg * c->ncg * c->ncg; c->ns = c->nc * c->ncg * c->ncg; c->nf = c->ncg * c->ncg;
+    c->params = *prm;
+    // topology checks: every element face in exactly one mortar, orientation 0
+    std::vector<int> e2m((size_t)c->K * 6, -1);
+    std::vector<int> me(2 * (size_t)c->M), mf(2 * (size_t)c->M), msg(c->M, 1), mns(c->M, 1);
+    for (int k = 0; k < c->M; ++k) {
+        if (mesh->mortar_orient && mesh->mortar_orient[k] != 0)
+            return set_err(c, DSEM_ERR_UNSUPPORTED, "rotated mortar faces (orientation != 0) are not supported");
+        for (int s = 0; s < 2; ++s) {
+            const int e = mesh->mortar_elem[2 * k + s], f = mesh->mortar_face[2 * k + s];
+            if (e < 0 || e >= c->K || f < 1 || f > 6) return set_err(c, DSEM_ERR_ARG, "mortar element/face out of range");
+            if (e2m[(size_t)e * 6 + f - 1] != -1) return set_err(c, DSEM_ERR_ARG, "element face in two mortars");
+            e2m[(size_t)e * 6 + f - 1] = (k << 1) | s;
+            me[2 * k + s] = e;
+            mf[2 * k + s] = f;
+        }
+        if (mesh->mortar_sign) msg[k] = mesh->mortar_sign[k];
+        if (mesh->mortar_nsign) mns[k] = mesh->mortar_nsign[k];
+        if ((msg[k] != 1 && msg[k] != -1) || (mns[k] != 1 && mns[k] != -1))
+            return set_err(c, DSEM_ERR_ARG, "mortar sign/nsign must be +-1");
+    }
+    for (size_t i = 0; i < e2m.size(); ++i)
+        if (e2m[i] < 0) return set_err(c, DSEM_ERR_UNSUPPORTED, "open element face: boundary conditions are not part of this path");
+
+    HIP_TRY(c, hipSetDevice(device));
+    HIP_TRY(c, hipStreamCreate(&c->stream));
+    HIP_TRY(c, hipEventCreate(&c->ev0));
+    HIP_TRY(c, hipEventCreate(&c->ev1));
+    TRY(upload_operators(c));
+    if (!dsem::make_ops(c->p, c->ops)) return set_err(c, DSEM_ERR_ARG, "operators");
+
+    dsem::Geometry g;
+    dsem::element_geometry(c->ops, c->K, mesh->corners, g);
+    std::vector<double> nhat, norm;
+    dsem::mortar_geometry(c->ops, c->M, mesh->corners, mesh->mortar_elem, mesh->mortar_face, nhat, norm);
+    c->h_xg = g.xg;
+    c->h_jacob = g.jacob;
+    int mode = prm->metrics;
+    if (mode == DSEM_METRICS_AUTO) mode = g.affine ? DSEM_METRICS_PER_ELEMENT : DSEM_METRICS_PER_POINT;
+    if (mode == DSEM_METRICS_PER_ELEMENT && !g.affine)
+        return set_err(c, DSEM_ERR_ARG, "per-element metrics need parallelepiped elements");
+    c->ms = (mode == DSEM_METRICS_PER_ELEMENT) ? 0 : 1;
+    if (c->ms == 0) {
+        // keep the value at the first point of every element / mortar
+        auto compact = [](std::vector<double> &v, size_t groups, size_t stride) {
+            std::vector<double> o(groups);
+            for (size_t i = 0; i < groups; ++i) o[i] = v[i * stride];
+            v.swap(o);
+        };
+        compact(g.jacob, (size_t)c->K, c->np);
+        compact(g.gmetg, (size_t)c->K * 9, c->np);
+        for (int d = 0; d < 3; ++d) compact(g.gmet[d], (size_t)c->K * 3, c->ns);
+        compact(nhat, (size_t)c->M * 3, c->nf);
+        compact(norm, (size_t)c->M, c->nf);
+    }
+    TRY(upload(c, &c->jacob, g.jacob));
+    TRY(upload(c, &c->gmetg, g.gmetg));
+    for (int d = 0; d < 3; ++d) TRY(upload(c, &c->gmet[d], g.gmet[d]));
+    TRY(upload(c, &c->nhat, nhat));
+    TRY(upload(c, &c->norm, norm));
+    TRY(upload(c, &c->e2m, e2m));
+    TRY(upload(c, &c->m_elem, me));
+    TRY(upload(c, &c->m_face, mf));
+    TRY(upload(c, &c->m_sign, msg));
+    TRY(upload(c, &c->m_nsign, mns));
+    const size_t nst = (size_t)c->K * NEQ * c->np;
+    TRY(dev_alloc(c, &c->JQ, nst));
+    TRY(dev_alloc(c, &c->G, nst));
+    TRY(dev_alloc(c, &c->Qtp, nst));
+    TRY(dev_alloc(c, &c->Qt, nst));
+    TRY(dev_alloc(c, &c->Gr, (size_t)c->K * 15 * c->np));
+    TRY(dev_alloc(c, &c->T, (size_t)c->K * 6 * NEQ * c->nf));
+    TRY(dev_alloc(c, &c->FV, (size_t)c->K * 6 * NEQ * c->nf));
+    TRY(dev_alloc(c, &c->QA, (size_t)c->M * NEQ * c->nf));
+    TRY(dev_alloc(c, &c->FI, (size_t)c->M * NEQ * c->nf));
+    TRY(dev_alloc(c, &c->FT, (size_t)c->M * NEQ * c->nf));
+    TRY(dev_alloc(c, &c->muh, (size_t)c->K * 3 * c->ns));
+    TRY(dev_alloc(c, &c->d_scalars, 4));
+    HIP_TRY(c, hipMemset(c->JQ, 0, sizeof(double) * nst));
+    HIP_TRY(c, hipMemset(c->G, 0, sizeof(double) * nst));
+    HIP_TRY(c, hipMemset(c->muh, 0, sizeof(double) * (size_t)c->K * 3 * c->ns));
+    return DSEM_OK;
+}
+
+void pack_state(const dsem_ctx *c, const double *in, int layout, double *out)
+{
+    // layout 1 (reference padded 10^3 per variable) -> compact
+    const int ncg = c->ncg, np = c->np;
+    for (int e = 0; e < c->K; ++e)
+        for (int v = 0; v < NEQ; ++v)
+            for (int l = 0; l < ncg; ++l)
+                for (int m = 0; m < ncg; ++m)
+                    for (int n = 0; n < ncg; ++n) {
+                        const size_t src = (((size_t)e * NEQ + v) * 10 + l) * 100 + (size_t)m * 10 + n;
+                        const size_t dst = ((size_t)e * NEQ + v) * np + n + ncg * (m + ncg * l);
+                        if (layout == 1) out[dst] = in[src];
+                        else out[src] = in[dst];
+                    }
+}
+
+}  // namespace
+
+// ====================================================================== C ABI
+extern "C" {
+
+int dsem_create(const dsem_mesh_desc *mesh, const dsem_params *params, int device, dsem_ctx **out)
+{
+    if (!out) return DSEM_ERR_ARG;
+    *out = nullptr;
+    dsem_ctx *c = new (std::nothrow) dsem_ctx();
+    if (!c) return DSEM_ERR_NOMEM;
+    int rc = create_impl(mesh, params, device, c);
+    *out = c;   // returned even on failure so that dsem_last_error can report; caller destroys
+    return rc;
+}
+
+int dsem_destroy(dsem_ctx *c)
+{
+    if (!c) return DSEM_OK;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (void *p : c->allocations) (void)hipFree(p);
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return DSEM_OK;
+}
+
+const char *dsem_last_error(const dsem_ctx *c)
+{
+    return c ? c->err.c_str() : "null context";
+}
+
+int dsem_sizes(const dsem_ctx *c, int32_t *K, int32_t *p, int32_t *nmortar, int64_t *dof)
+{
+    if (!c) return DSEM_ERR_ARG;
+    if (K) *K = c->K;
+    if (p) *p = c->p;
+    if (nmortar) *nmortar = c->M;
+    if (dof) *dof = (int64_t)NEQ * c->K * c->np;
+    return DSEM_OK;
+}
+
+int dsem_gauss_points(dsem_ctx *c, double *xg, double *jacob)
+{
+    if (!c || !c->stream) return DSEM_ERR_ARG;
+    if (xg) std::memcpy(xg, c->h_xg.data(), sizeof(double) * c->h_xg.size());
+    if (jacob) std::memcpy(jacob, c->h_jacob.data(), sizeof(double) * c->h_jacob.size());
+    return DSEM_OK;
+}
+
+int dsem_set_state(dsem_ctx *c, const double *JQ, int layout)
+{
+    if (!c || !c->stream || !JQ || (layout != 0 && layout != 1)) return set_err(c, DSEM_ERR_ARG, "set_state arguments");
+    const size_t n = (size_t)c->K * NEQ * c->np;
+    std::vector<double> tmp;
+    const double *src = JQ;
+    if (layout == 1) {
+        tmp.resize(n);
+        pack_state(c, JQ, 1, tmp.data());
+        src = tmp.data();
+    }
+    HIP_TRY(c, hipSetDevice(c->device));
+    HIP_TRY(c, hipMemcpyAsync(c->JQ, src, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return DSEM_OK;
+}
+
+int dsem_get_state(dsem_ctx *c, double *JQ, int layout)
+{
+    if (!c || !c->stream || !JQ || (layout != 0 && layout != 1)) return set_err(c, DSEM_ERR_ARG, "get_state arguments");
+    const size_t n = (size_t)c->K * NEQ * c->np;
+    HIP_TRY(c, hipSetDevice(c->device));
+    if (layout == 0) {
+        HIP_TRY(c, hipMemcpyAsync(JQ, c->JQ, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(c, hipStreamSynchronize(c->stream));
+        return DSEM_OK;
+    }
+    std::vector<double> tmp(n);
+    HIP_TRY(c, hipMemcpyAsync(tmp.data(), c->JQ, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    std::memset(JQ, 0, sizeof(double) * (size_t)c->K * NEQ * 1000);
+    pack_state(c, tmp.data(), 0, JQ);
+    return DSEM_OK;
+}
+
+int dsem_set_viscosity(dsem_ctx *c, const double *muh)
+{
+    if (!c || !c->stream || !muh) return set_err(c, DSEM_ERR_ARG, "set_viscosity arguments");
+    HIP_TRY(c, hipSetDevice(c->device));
+    HIP_TRY(c, hipMemcpyAsync(c->muh, muh, sizeof(double) * (size_t)c->K * 3 * c->ns, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return DSEM_OK;
+}
+
+int dsem_rhs_device(dsem_ctx *c, const double *dJQ, double *dQt)
+{
+    if (!c || !c->stream || !dJQ || !dQt) return set_err(c, DSEM_ERR_ARG, "rhs_device arguments");
+    HIP_TRY(c, hipSetDevice(c->device));
+    StageArgs st{0.0, 0.0, 0, 0, nullptr, nullptr};
+    return rhs_dispatch(c, const_cast<double *>(dJQ), st, dQt);
+}
+
+int dsem_rhs(dsem_ctx *c, double t, double *Qt_out)
+{
+    (void)t;   // no time-dependent source terms on this path (source terms = none)
+    if (!c || !c->stream) return set_err(c, DSEM_ERR_ARG, "rhs arguments");
+    HIP_TRY(c, hipSetDevice(c->device));
+    StageArgs st{0.0, 0.0, 0, 0, nullptr, nullptr};
+    TRY(rhs_dispatch(c, c->JQ, st, c->Qt));
+    if (Qt_out) {
+        HIP_TRY(c, hipMemcpyAsync(Qt_out, c->Qt, sizeof(double) * (size_t)c->K * NEQ * c->np, hipMemcpyDeviceToHost,
+                                  c->stream));
+    }
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return DSEM_OK;
+}
+
+int dsem_rk_steps(dsem_ctx *c, double t, double dt, int nsteps, double *resid_out)
+{
+    (void)t;
+    if (!c || !c->stream || nsteps < 1) return set_err(c, DSEM_ERR_ARG, "rk_steps arguments");
+    static const double ark[5] = {0.0, -0.4178904745, -1.192151694643, -1.697784692471, -1.514183444257};
+    static const double crk[5] = {0.1496590219993, 0.3792103129999, 0.8229550293869, 0.6994504559488, 0.1530572479681};
+    HIP_TRY(c, hipSetDevice(c->device));
+    const bool check = resid_out != nullptr;
+    if (check) HIP_TRY(c, hipMemsetAsync(c->d_scalars, 0, sizeof(unsigned long long) * 4, c->stream));
+    for (int s = 0; s < nsteps; ++s) {
+        for (int k = 0; k < 5; ++k) {
+            StageArgs st;
+            st.a = ark[k];
+            st.cdt = crk[k] * dt;
+            st.first = (k == 0);
+            st.rk = 1;
+            const bool last = check && s == nsteps - 1 && k == 4;
+            st.resid = last ? &c->d_scalars[0] : nullptr;
+            st.nonfinite = check ? reinterpret_cast<unsigned int *>(&c->d_scalars[2]) : nullptr;
+            TRY(rhs_dispatch(c, c->JQ, st, nullptr));
+        }
+    }
+    if (check) {
+        unsigned long long h[4];
+        HIP_TRY(c, hipMemcpyAsync(h, c->d_scalars, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(c, hipStreamSynchronize(c->stream));
+        double r;
+        std::memcpy(&r, &h[0], sizeof(double));
+        *resid_out = r;
+        if (h[2] & 0xffffffffull) return set_err(c, DSEM_ERR_NONFINITE, "NaN or Inf in the residual");
+    } else {
+        HIP_TRY(c, hipStreamSynchronize(c->stream));
+    }
+    return DSEM_OK;
+}
+
+int dsem_rk_step(dsem_ctx *c, double t, double dt, double *resid_out)
+{
+    return dsem_rk_steps(c, t, dt, 1, resid_out);
+}
+
+int dsem_dtcalc(dsem_ctx *c, double *dt_out)
+{
+    if (!c || !c->stream || !dt_out) return set_err(c, DSEM_ERR_ARG, "dtcalc arguments");
+    HIP_TRY(c, hipSetDevice(c->device));
+    const double big = 1.0e308;
+    unsigned long long init;
+    std::memcpy(&init, &big, sizeof(init));
+    HIP_TRY(c, hipMemcpyAsync(&c->d_scalars[1], &init, sizeof(init), hipMemcpyHostToDevice, c->stream));
+    TRY(dtcalc_dispatch(c, &c->d_scalars[1]));
+    unsigned long long h;
+    HIP_TRY(c, hipMemcpyAsync(&h, &c->d_scalars[1], sizeof(h), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    std::memcpy(dt_out, &h, sizeof(double));
+    return DSEM_OK;
+}
+
+int dsem_state_pointers(dsem_ctx *c, double **dJQ, double **dG)
+{
+    if (!c) return DSEM_ERR_ARG;
+    if (dJQ) *dJQ = c->JQ;
+    if (dG) *dG = c->G;
+    return DSEM_OK;
+}
+
+int dsem_stream(dsem_ctx *c, void **s)
+{
+    if (!c || !s) return DSEM_ERR_ARG;
+    *s = c->stream;
+    return DSEM_OK;
+}
+
+int dsem_synchronize(dsem_ctx *c)
+{
+    if (!c || !c->stream) return DSEM_ERR_ARG;
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return DSEM_OK;
+}
+
+int dsem_timing_enable(dsem_ctx *c, int enable)
+{
+    if (!c) return DSEM_ERR_ARG;
+    c->timing = enable != 0;
+    return DSEM_OK;
+}
+
+int dsem_timing_reset(dsem_ctx *c)
+{
+    if (!c) return DSEM_ERR_ARG;
+    for (int i = 0; i < KID_COUNT; ++i) { c->t_ms[i] = 0.0; c->t_n[i] = 0; }
+    return DSEM_OK;
+}
+
+int dsem_timing_count(void) { return KID_COUNT; }
+
+int dsem_timing_query(const dsem_ctx *c, int kid, const char **name, double *total_ms, int64_t *launches)
+{
+    if (!c || kid < 0 || kid >= KID_COUNT) return DSEM_ERR_ARG;
+    if (name) *name = kKernelNames[kid];
+    if (total_ms) *total_ms = c->t_ms[kid];
+    if (launches) *launches = c->t_n[kid];
+    return DSEM_OK;
+}
+
+}  // extern "C"

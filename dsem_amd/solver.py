@@ -1,0 +1,166 @@
+"""Host-side mirror of the reference's time-stepping interface on the
+MI355X library.
+
+``DSEM.tstep(time, dt)`` stands where the driver calls
+``tstep(time,dt,d,ngrid,mrtr,nmort,resid)`` (src/tstep.f90:12,
+src/3DDriver_parfmdf.f90:466); ``DSEM.rhs`` is ``Fluxes`` followed by the
+divergence of ``update`` (fluxesH_newa.f90:17, updatea.f90:100-113);
+``DSEM.dtcalc`` is the driver's time-step computation
+(3DDriver_parfmdf.f90:362-370 with dtcalc.f90:9-58).  A NaN/Inf residual
+raises, where the reference writes NAN.dat and aborts
+(3DDriver_parfmdf.f90:489-499).
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib as L
+from .mesh import Mesh
+
+_PREC = {"fp64": L.PREC_FP64, "fp32": L.PREC_FP32, "bf16x3": L.PREC_BF16X3}
+_METRICS = {"auto": L.METRICS_AUTO, "per_point": L.METRICS_PER_POINT, "per_element": L.METRICS_PER_ELEMENT}
+
+
+class NonFiniteResidual(L.DsemError):
+    pass
+
+
+def _dp(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _ip(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_int32))
+
+
+class DSEM:
+    def __init__(self, mesh: Mesh, gamma: float = 1.4, re: float = 1600.0, pr: float = 0.71,
+                 mach: float = 1.0, shock: bool = False, precision: str = "fp64",
+                 metrics: str = "auto", device: int = 0, cmu: float = 1.0, mumax: float = 1.0,
+                 cmax: float = 0.5):
+        self._L = L.load()
+        self.mesh = mesh
+        self._keep = [np.ascontiguousarray(mesh.mortar_elem, dtype=np.int32),
+                      np.ascontiguousarray(mesh.mortar_face, dtype=np.int32),
+                      np.ascontiguousarray(mesh.mortar_sign, dtype=np.int32),
+                      np.ascontiguousarray(mesh.mortar_nsign, dtype=np.int32),
+                      np.ascontiguousarray(mesh.corners, dtype=np.float64)]
+        me, mf, ms, mn, cr = self._keep
+        desc = L.MeshDesc(mesh.K, mesh.p, mesh.nmortar, 0, _ip(me), _ip(mf), _ip(ms), _ip(mn), None, _dp(cr))
+        prm = L.Params(gamma, re, pr, mach, int(bool(shock)), _PREC[precision], _METRICS[metrics], 0,
+                       cmu, mumax, cmax)
+        h = C.c_void_p()
+        rc = self._L.dsem_create(C.byref(desc), C.byref(prm), device, C.byref(h))
+        self._h = h
+        if rc != L.DSEM_OK:
+            msg = self._L.dsem_last_error(h).decode() if h.value else "create failed"
+            self._L.dsem_destroy(h)
+            self._h = C.c_void_p()
+            raise L.DsemError(rc, msg)
+        self.K, self.p = mesh.K, mesh.p
+        self.ncg, self.nc = mesh.p + 1, mesh.p + 2
+        self.np, self.ns = self.ncg ** 3, self.nc * self.ncg ** 2
+        self.dof = 5 * self.K * self.np
+        self.params = dict(gamma=gamma, re=re, pr=pr, mach=mach, shock=shock, precision=precision)
+
+    # ------------------------------------------------------------------ util
+    def _check(self, rc: int):
+        if rc == L.DSEM_OK:
+            return
+        msg = self._L.dsem_last_error(self._h).decode()
+        if rc == L.DSEM_ERR_NONFINITE:
+            raise NonFiniteResidual(rc, msg)
+        raise L.DsemError(rc, msg)
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            self._L.dsem_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # ------------------------------------------------------------------ data
+    def gauss_points(self):
+        xg = np.zeros((self.K, self.np, 3))
+        jac = np.zeros((self.K, self.np))
+        self._check(self._L.dsem_gauss_points(self._h, _dp(xg), _dp(jac)))
+        return xg, jac
+
+    def set_state(self, JQ: np.ndarray, layout: int = 0):
+        shape = (self.K, 5, self.np) if layout == 0 else (self.K, 5, 10, 10, 10)
+        JQ = np.ascontiguousarray(JQ, dtype=np.float64)
+        if JQ.size != int(np.prod(shape)):
+            raise ValueError(f"state must have {int(np.prod(shape))} values for layout {layout}")
+        self._check(self._L.dsem_set_state(self._h, _dp(JQ), layout))
+
+    def get_state(self, layout: int = 0) -> np.ndarray:
+        shape = (self.K, 5, self.np) if layout == 0 else (self.K, 5, 10, 10, 10)
+        out = np.zeros(shape)
+        self._check(self._L.dsem_get_state(self._h, _dp(out), layout))
+        return out
+
+    def set_viscosity(self, muh: np.ndarray):
+        muh = np.ascontiguousarray(muh, dtype=np.float64)
+        if muh.size != self.K * 3 * self.ns:
+            raise ValueError("muh must be [K][3][nc*ncg^2]")
+        self._check(self._L.dsem_set_viscosity(self._h, _dp(muh)))
+
+    # ------------------------------------------------------------------ path
+    def rhs(self, t: float = 0.0) -> np.ndarray:
+        out = np.zeros((self.K, 5, self.np))
+        self._check(self._L.dsem_rhs(self._h, t, _dp(out)))
+        return out
+
+    def tstep(self, time: float, dt: float) -> float:
+        r = C.c_double(0.0)
+        self._check(self._L.dsem_rk_step(self._h, time, dt, C.byref(r)))
+        return r.value
+
+    def steps(self, n: int, dt: float, time: float = 0.0, check: bool = True) -> float | None:
+        if check:
+            r = C.c_double(0.0)
+            self._check(self._L.dsem_rk_steps(self._h, time, dt, n, C.byref(r)))
+            return r.value
+        self._check(self._L.dsem_rk_steps(self._h, time, dt, n, None))
+        return None
+
+    def dtcalc(self, cfl: float = 1.0) -> float:
+        d = C.c_double(0.0)
+        self._check(self._L.dsem_dtcalc(self._h, C.byref(d)))
+        return d.value * cfl
+
+    def synchronize(self):
+        self._check(self._L.dsem_synchronize(self._h))
+
+    def stream_handle(self) -> int:
+        s = C.c_void_p()
+        self._check(self._L.dsem_stream(self._h, C.byref(s)))
+        return s.value or 0
+
+    # ------------------------------------------------------------------ timing
+    def timing(self, enable: bool):
+        self._check(self._L.dsem_timing_enable(self._h, int(enable)))
+        if enable:
+            self._check(self._L.dsem_timing_reset(self._h))
+
+    def timing_report(self) -> dict:
+        out = {}
+        for k in range(self._L.dsem_timing_count()):
+            name = C.c_char_p()
+            ms = C.c_double()
+            n = C.c_int64()
+            self._check(self._L.dsem_timing_query(self._h, k, C.byref(name), C.byref(ms), C.byref(n)))
+            out[name.value.decode()] = (ms.value, n.value)
+        return out

@@ -1,0 +1,146 @@
+/*
+ * dsem_hip.h -- C ABI of the MI355X (gfx950) DSEM right-hand side and
+ * low-storage RK4 step.
+ *
+ * The reference drives this path from Fortran with implicit interfaces and
+ * derived types (`TYPE(domain) d(ngp)`, `TYPE(mortar) mrtr(nmp)`), neither of
+ * which is C-interoperable, so the drop-in takes flat arrays:
+ *
+ *   reference                                            replaced by
+ *   tstep(time,dt,d,ngrid,mrtr,nmort,resid)              dsem_rk_step
+ *       src/tstep.f90:12 (called at src/3DDriver_parfmdf.f90:466)
+ *   Fluxes(d,ngrid,mrtr,nmort,time) + update's           dsem_rhs
+ *       divergence  src/Parabolic3D/stac3m/fluxesH_newa.f90:17,
+ *                   src/stac3m/updatea.f90:100-113
+ *   dtcalc(dt,d)  src/Parabolic3D/NS/NS/stac3m/dtcalc.f90:9 dsem_dtcalc
+ *   set_points/set_matrices/set_geometry/set_mortar_nsign dsem_create
+ *       src/Grid_Routines/stac3m_geomLib.f90:7,
+ *       src/Spectral/stac3m/matrices.f90:20,
+ *       src/Grid_Routines/set_domain_geometry.f90:8,
+ *       src/Grid_Routines/set_mortars.f90:227
+ *   d(id)%Q gather/scatter (Q stored as J*Q,             dsem_set_state / dsem_get_state
+ *       src/3DDriver_parfmdf.f90:286-300)
+ *   physics / rk_coefs module globals                    dsem_params
+ *       src/paraFMDF/NS_data.f90:5-40, initializationsc2.f90:34-40
+ *   Is_NaN/Is_INF check of resid                         DSEM_ERR_NONFINITE
+ *       src/3DDriver_parfmdf.f90:489-499
+ *
+ * Conventions: every call returns 0 on success or a negative DSEM_ERR_*
+ * code (message in dsem_last_error).  The context owns all device memory;
+ * host buffers are only read or written during the call.  Calls are
+ * synchronous with respect to the host unless named *_async.  One context
+ * serves one device; a context is not thread-safe.
+ *
+ * State layout: J-weighted conserved variables (rho, rho u, rho v, rho w,
+ * rho e) times J, FP64.
+ *   layout 0  compact   [K][5][ncg^3], point n + ncg*(m + ncg*l)
+ *   layout 1  reference [K][5][10][10][10], i.e. d(id)%Q(nx,ny,nz,neq) with
+ *             nx=ny=nz=10 (caseFiles/tgv/size.f90), padding ignored on input
+ *             and zero on output.
+ */
+#ifndef DSEM_HIP_H
+#define DSEM_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DSEM_OK               0
+#define DSEM_ERR_ARG         -1
+#define DSEM_ERR_HIP         -2
+#define DSEM_ERR_NONFINITE   -3
+#define DSEM_ERR_UNSUPPORTED -4
+#define DSEM_ERR_NOMEM       -5
+
+#define DSEM_PREC_FP64   0   /* plain FP64 drop-in                           */
+#define DSEM_PREC_FP32   1   /* FP32 contractions, FP64 state/RK/reductions  */
+#define DSEM_PREC_BF16X3 2   /* BF16 3-term split contractions, FP32 accum   */
+
+#define DSEM_METRICS_AUTO        0  /* per element when every element is affine */
+#define DSEM_METRICS_PER_POINT   1  /* reference: metrics at every point         */
+#define DSEM_METRICS_PER_ELEMENT 2  /* affine (parallelepiped) elements only     */
+
+typedef struct dsem_ctx dsem_ctx;
+
+/* Conforming hex8 mesh.  Mortar side 0 is the master face (the one whose
+ * geometry defines the normal, as in set_domain_geometry.f90:254-345).
+ * Faces use the reference numbering 1..6 (1: eta=0, 2: eta=1, 3: zeta=0,
+ * 4: xi=1, 5: zeta=1, 6: xi=0).  Only orientation 0 (aligned faces) is
+ * supported. */
+typedef struct {
+    int32_t K;                     /* elements                               */
+    int32_t p;                     /* order: ncg = p+1 Gauss, nc = p+2 Lobatto, 1..8 */
+    int32_t nmortar;
+    int32_t reserved;
+    const int32_t *mortar_elem;    /* [nmortar][2] 0-based element ids       */
+    const int32_t *mortar_face;    /* [nmortar][2] 1..6                       */
+    const int32_t *mortar_sign;    /* [nmortar] +-1 (slave flux sign) or NULL */
+    const int32_t *mortar_nsign;   /* [nmortar] +-1 (normal sign) or NULL     */
+    const int32_t *mortar_orient;  /* [nmortar] must be 0, or NULL            */
+    const double  *corners;        /* [K][8][3] hex8 corners, FE order        */
+} dsem_mesh_desc;
+
+typedef struct {
+    double  gamma, re, pr, mach;   /* physics module (mach = 1 with T0=100 gives Ma 0.1) */
+    int32_t shock;                 /* entropy viscosity added to mu (0/1)     */
+    int32_t precision;             /* DSEM_PREC_*                             */
+    int32_t metrics;               /* DSEM_METRICS_*                          */
+    int32_t reserved;
+    double  cmu, mumax, cmax;      /* entropy-viscosity coefficients          */
+} dsem_params;
+
+int  dsem_create(const dsem_mesh_desc *mesh, const dsem_params *params, int device, dsem_ctx **out);
+int  dsem_destroy(dsem_ctx *ctx);
+const char *dsem_last_error(const dsem_ctx *ctx);
+
+/* Problem sizes: K, p, nmortar, degrees of freedom (5*K*ncg^3). */
+int  dsem_sizes(const dsem_ctx *ctx, int32_t *K, int32_t *p, int32_t *nmortar, int64_t *dof);
+
+/* Gauss-point coordinates [K][ncg^3][3] and jacob (=1/J) [K][ncg^3]
+ * (set_domain_geometry.f90:86-97); either pointer may be NULL. */
+int  dsem_gauss_points(dsem_ctx *ctx, double *xg, double *jacob);
+
+int  dsem_set_state(dsem_ctx *ctx, const double *JQ, int layout);
+int  dsem_get_state(dsem_ctx *ctx, double *JQ, int layout);
+
+/* Entropy viscosity mu_h on the three staggered grids, [K][3][nc*ncg^2]
+ * (muhlgg/muhglg/muhggl of Vis_Fluxes_new.f90:126-190), used when shock=1. */
+int  dsem_set_viscosity(dsem_ctx *ctx, const double *muh);
+
+/* Qt = d(JQ)/dt for the current state (Fluxes + update's divergence);
+ * Qt_out [K][5][ncg^3] compact, or NULL to keep it on the device. */
+int  dsem_rhs(dsem_ctx *ctx, double t, double *Qt_out);
+
+/* One Carpenter-Kennedy RK4(5) 2N step (tstep).  resid_out (nullable)
+ * receives max|Qt| over the last element in the last stage, as the
+ * reference's resid; DSEM_ERR_NONFINITE is returned when any stage produced
+ * a NaN or Inf anywhere (checked only when resid_out is non-NULL). */
+int  dsem_rk_step(dsem_ctx *ctx, double t, double dt, double *resid_out);
+
+/* Several steps back to back without host synchronisation in between. */
+int  dsem_rk_steps(dsem_ctx *ctx, double t, double dt, int nsteps, double *resid_out);
+
+/* Inviscid time-step bound before the CFL factor (dtcalc over all elements). */
+int  dsem_dtcalc(dsem_ctx *ctx, double *dt_out);
+
+/* Device-pointer entry points (same HIP device, stream-ordered on the
+ * context's stream): dJQ and dQt are [K][5][ncg^3] FP64 device buffers. */
+int  dsem_rhs_device(dsem_ctx *ctx, const double *dJQ, double *dQt);
+int  dsem_state_pointers(dsem_ctx *ctx, double **dJQ, double **dG);
+int  dsem_stream(dsem_ctx *ctx, void **hip_stream);
+int  dsem_synchronize(dsem_ctx *ctx);
+
+/* Per-kernel timing with HIP events on the context's stream.  When enabled,
+ * every launch is bracketed by events and synchronised, so totals are
+ * kernel durations; use it for profiling runs only. */
+int  dsem_timing_enable(dsem_ctx *ctx, int enable);
+int  dsem_timing_reset(dsem_ctx *ctx);
+int  dsem_timing_count(void);
+int  dsem_timing_query(const dsem_ctx *ctx, int kernel_id, const char **name, double *total_ms, int64_t *launches);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* DSEM_HIP_H */
