@@ -47,12 +47,13 @@ def kernel_work(n_elem: int, nmort: int, p: int) -> dict:
     w = {}
     w["k_traces"] = dict(bytes=K * (5 * npt + 6 * 5 * nf) * d, flops=K * 6 * 5 * nf * 2 * ncg)
     w["k_face_inv"] = dict(bytes=Mm * nf * (10 + 10) * d, flops=Mm * nf * 120)
-    w["k_grad"] = dict(bytes=K * (5 * npt + 6 * 5 * nf + 15 * npt) * d,
-                       flops=K * 5 * npt * 3 * 2 * (ncg + 2) + K * 15 * npt * 5)
+    # gradients (combined D_int*B operator plus two face terms per direction),
     # interpolation of Q to 3 grids, prolongation of 15 gradients to 3 grids,
-    # interior divergence, plus ~260 flops of pointwise Euler+viscous flux per point
-    w["k_volume"] = dict(bytes=K * (5 * npt + 15 * npt + 6 * 5 * nf + 6 * 5 * nf + 5 * npt) * d,
-                         flops=K * (3 * 5 * ns * 2 * ncg + 3 * 15 * ns * 2 * ncg + 3 * 5 * npt * 2 * (nc - 2)
+    # interior divergence, plus ~260 flops of pointwise Euler+viscous flux per point.
+    # Bytes: read Q, the face averages of the 6 faces; write FV and Qtp.
+    w["k_volume"] = dict(bytes=K * (5 * npt + 6 * 5 * nf + 6 * 5 * nf + 5 * npt) * d,
+                         flops=K * (5 * npt * 3 * 2 * (ncg + 2) + 15 * npt * 5
+                                    + 3 * 5 * ns * 2 * ncg + 3 * 15 * ns * 2 * ncg + 3 * 5 * npt * 2 * (nc - 2)
                                     + 3 * ns * 260))
     w["k_face_visc"] = dict(bytes=Mm * nf * (10 + 5 + 5) * d, flops=Mm * nf * 5 * 6)
     w["k_update"] = dict(bytes=K * npt * 5 * (1 + 2 + 2) * d + Mm * nf * 5 * d * 2, flops=K * npt * 5 * 16)

@@ -52,5 +52,30 @@ def build(verbose: bool = False) -> str:
     return LIB
 
 
+
+
+def build_fortran(verbose: bool = False) -> str:
+    """Compile the iso_c_binding module and the Fortran test driver
+    (tests/fortran/tgv_drive) against libdsem_hip.so with flang."""
+    flang = os.environ.get("FLANG", "/opt/rocm/lib/llvm/bin/flang")
+    if not os.path.exists(flang):
+        raise RuntimeError("flang not found")
+    fdir = os.path.join(HERE, "fortran")
+    tdir = os.path.join(ROOT, "tests", "fortran")
+    obj = os.path.join(fdir, "dsem_hip.o")
+    exe = os.path.join(tdir, "tgv_drive")
+    cmds = [[flang, "-O2", "-c", os.path.join(fdir, "dsem_hip.f90"), "-module-dir", fdir, "-o", obj],
+            [flang, "-O2", "-I", fdir, "-module-dir", tdir, os.path.join(tdir, "tgv_drive.f90"), obj,
+             "-L", HERE, "-ldsem_hip", "-Wl,-rpath,$ORIGIN/../../dsem_amd", "-o", exe]]
+    for cmd in cmds:
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        out = subprocess.run(cmd, capture_output=True, text=True)
+        if out.returncode != 0:
+            raise RuntimeError("flang failed:\n" + out.stdout + out.stderr)
+    return exe
+
+
 if __name__ == "__main__":
     print(build(verbose=True))
+    print(build_fortran(verbose=True))

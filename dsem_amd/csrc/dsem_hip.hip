@@ -11,11 +11,11 @@
 //                   face norm and the averaged state (interface_flux,
 //                   mortar_operations.f90:151-238; riemann, PhysLib.f90:231-320;
 //                   V_Avg_Soln, mortar_operations.f90:781-837)
-//   k_grad          Gauss-point gradients of Q from the staggered states whose
+//   k_volume        Gauss-point gradients of Q from the staggered states whose
 //                   face values are the mortar averages (Gauss_Deriv,
-//                   SolnDerivsc.f90:14-83), using the combined operator
-//                   D_int*B for the interior Lobatto points
-//   k_volume        per staggered grid: interpolated state, prolonged
+//                   SolnDerivsc.f90:14-83, with the combined operator D_int*B
+//                   for the interior Lobatto points), kept in LDS; then per
+//                   staggered grid: interpolated state, prolonged
 //                   gradients, Euler + viscous contravariant fluxes at
 //                   interior Lobatto points, viscous face traces r/norm, and
 //                   the interior part of the flux divergence
@@ -30,7 +30,6 @@
 //
 // Element-major structure-of-arrays layouts (ncg = p+1, nc = p+2):
 //   JQ, G, Qtp [K][5][np]        point n + ncg*(m + ncg*l)
-//   Gr         [K][15][np]       (c*5 + v): dQ_v/dx_c at Gauss points
 //   T, FV      [K][6][5][nf]     face traces, face f-1, point i + ncg*j
 //   QA, FI, FT [M][5][nf]        per mortar (master face indexing)
 //   metrics    jacob [K][npm], gmetg [K][9][npm], gmet_d [K][3][nsm],
@@ -51,6 +50,7 @@ namespace {
 
 constexpr int NEQ = 5;
 constexpr int BLOCK = 256;
+constexpr int VBLOCK = 256;   // fused volume kernel: one element per workgroup, one wave per SIMD
 
 // operators for every order p = 1..8, indexed [p]
 __constant__ double c_D[9][90];   // D[k*nc + j]
@@ -144,6 +144,61 @@ __device__ __forceinline__ void viscous_contravariant(const DevPhys &ph, const d
     const double f4 = u * txx + v * txy + w * txz + hfac * tx;
     const double g4 = v * tyy + u * txy + w * tyz + hfac * ty;
     const double h4 = v * tyz + u * txz + w * tzz + hfac * tz;
+    r[0] = 0.0;
+    r[1] = g0 * txx + g1 * txy + g2 * txz;
+    r[2] = g0 * txy + g1 * tyy + g2 * tyz;
+    r[3] = g0 * txz + g1 * tyz + g2 * tzz;
+    r[4] = g0 * f4 + g1 * g4 + g2 * h4;
+}
+
+// The same viscous flux split in two steps so that the conservative
+// gradients of one direction can be consumed as soon as they are formed:
+// prim_gradient turns dQ/dx_c into (du/dx_c, dv/dx_c, dw/dx_c, dT/dx_c)
+// with the expressions of x/y/z_vis_flux (PhysLib.f90:448-462), and
+// viscous_from_primitive forms the stress/heat flux (PhysLib.f90:464-470).
+struct Prim {
+    double rho, u, v, w, q2, p;
+};
+
+__device__ __forceinline__ Prim primitives(const DevPhys &ph, const double *Q)
+{
+    Prim s;
+    s.rho = Q[0];
+    s.u = Q[1] / s.rho;
+    s.v = Q[2] / s.rho;
+    s.w = Q[3] / s.rho;
+    s.q2 = s.u * s.u + s.v * s.v + s.w * s.w;
+    s.p = (ph.gamma - 1.0) * (Q[4] - 0.5 * s.rho * s.q2);
+    return s;
+}
+
+__device__ __forceinline__ void prim_gradient(const DevPhys &ph, const Prim &s, const double *Qc, double *out)
+{
+    const double uc = (Qc[1] - s.u * Qc[0]) / s.rho;
+    const double vc = (Qc[2] - s.v * Qc[0]) / s.rho;
+    const double wc = (Qc[3] - s.w * Qc[0]) / s.rho;
+    const double pc = (ph.gamma - 1.0) * (Qc[4] - 0.5 * Qc[0] * s.q2 - s.rho * (s.u * uc + s.v * vc + s.w * wc));
+    out[0] = uc;
+    out[1] = vc;
+    out[2] = wc;
+    out[3] = (pc - s.p * Qc[0] / s.rho) * (ph.mach * ph.mach * ph.gamma) / s.rho;
+}
+
+// L[i][c] = d u_i / d x_c (i = u,v,w), T[c] = dT/dx_c
+__device__ __forceinline__ void viscous_from_primitive(const DevPhys &ph, const Prim &s, const double L[3][3],
+                                                       const double *T, double amu, double g0, double g1,
+                                                       double g2, double *r)
+{
+    const double gamma = ph.gamma, mach = ph.mach;
+    const double akappa = 1.0 + (amu - 1.0) * 2.0e-1 / (gamma - 1.0);
+    const double hfac = akappa / (ph.pr * mach * mach * (gamma - 1.0));
+    const double div = L[0][0] + L[1][1] + L[2][2];
+    const double txx = 2.0 * amu * (L[0][0] - div / 3.0), tyy = 2.0 * amu * (L[1][1] - div / 3.0);
+    const double tzz = 2.0 * amu * (L[2][2] - div / 3.0);
+    const double txy = amu * (L[0][1] + L[1][0]), txz = amu * (L[2][0] + L[0][2]), tyz = amu * (L[2][1] + L[1][2]);
+    const double f4 = s.u * txx + s.v * txy + s.w * txz + hfac * T[0];
+    const double g4 = s.v * tyy + s.u * txy + s.w * tyz + hfac * T[1];
+    const double h4 = s.v * tyz + s.u * txz + s.w * tzz + hfac * T[2];
     r[0] = 0.0;
     r[1] = g0 * txx + g1 * txy + g2 * txz;
     r[2] = g0 * txy + g1 * tyy + g2 * tyz;
@@ -265,147 +320,193 @@ __device__ __forceinline__ void load_physical_state(const double *__restrict__ J
     for (int i = threadIdx.x; i < NEQ * S::NP; i += BLOCK) sQ[i] = q[i] * jac[(i % S::NP) * m.ms];
 }
 
-template <int P>
-__global__ __launch_bounds__(BLOCK) void k_grad(const double *__restrict__ JQ, DevMesh m, const double *__restrict__ QA,
-                                                 double *__restrict__ Gr)
+// One staggered grid D of the fused volume kernel: fluxes at the nc*ncg^2
+// points of grid D into LDS (interior) or the viscous face traces (faces),
+// then the interior part of the divergence along D added into acc.
+template <int P, int D>
+__device__ __forceinline__ void volume_grid(const DevMesh &m, const DevPhys &ph, const double *__restrict__ QA,
+                                            const double *__restrict__ muh, double *__restrict__ FV,
+                                            const double *sQ, const double *sG, double *sF, int e,
+                                            const int *mort, double (&acc)[(Sz<P>::NP + VBLOCK - 1) / VBLOCK][5])
 {
     using S = Sz<P>;
-    __shared__ double sQ[NEQ * S::NP];
-    const int e = blockIdx.x;
-    load_physical_state<P>(JQ, m, e, sQ);
-    __syncthreads();
-    const int *e2m = m.e2m + (size_t)e * 6;
-    const int npm = m.ms ? S::NP : 1;
-    const double *gg = m.gmetg + (size_t)e * 9 * npm;
-    for (int i = threadIdx.x; i < NEQ * S::NP; i += BLOCK) {
-        const int v = i / S::NP, pt = i % S::NP;
-        const int n = pt % S::NCG, mi = (pt / S::NCG) % S::NCG, l = pt / (S::NCG * S::NCG);
-        const double *sv = sQ + v * S::NP;
-        double w[3];
+    constexpr int R = (S::NP + VBLOCK - 1) / VBLOCK;
+    const int tid = threadIdx.x;
+    const int nsm = m.ms ? S::NS : 1, nfm = m.ms ? S::NF : 1;
+    const double *gmd = (D == 0 ? m.gmet0 : (D == 1 ? m.gmet1 : m.gmet2)) + (size_t)e * 3 * nsm;
+    const double inv_re = 1.0 / ph.re;
+    __syncthreads();   // sG complete / previous divergence pass done with sF
+#pragma unroll 1
+    for (int s = tid; s < S::NS; s += VBLOCK) {
+        int j, a, b;
+        if (D == 0) { j = s % S::NC; a = (s / S::NC) % S::NCG; b = s / (S::NC * S::NCG); }
+        else if (D == 1) { a = s % S::NCG; j = (s / S::NCG) % S::NC; b = s / (S::NCG * S::NC); }
+        else { a = s % S::NCG; b = (s / S::NCG) % S::NCG; j = s / (S::NCG * S::NCG); }
+        const bool face = (j == 0 || j == S::NC - 1);
+        const int q = a + S::NCG * b;
+        double Bj[S::NCG];
 #pragma unroll
-        for (int d = 0; d < 3; ++d) {
-            const int k = d == 0 ? n : (d == 1 ? mi : l);
-            const int a = d == 0 ? mi : n;
-            const int b = d == 2 ? mi : l;
-            double s = 0.0;
+        for (int k = 0; k < S::NCG; ++k) Bj[k] = c_B[P][j * S::NCG + k];
+        const int fc = face_of<P>(D, j);
+        const int mo = mort[fc - 1];
+        double Qv[5];
+        if (face) {
 #pragma unroll
-            for (int k2 = 0; k2 < S::NCG; ++k2) s = fma(c_M[P][k * S::NCG + k2], sv[gidx<P>(d, k2, a, b)], s);
-            const int f0 = face_of<P>(d, 0), f1 = face_of<P>(d, 1);
-            const int q = a + S::NCG * b;
-            const double qa0 = QA[((size_t)(e2m[f0 - 1] >> 1) * NEQ + v) * S::NF + q];
-            const double qa1 = QA[((size_t)(e2m[f1 - 1] >> 1) * NEQ + v) * S::NF + q];
-            s = fma(c_D[P][k * S::NC + 0], qa0, s);
-            s = fma(c_D[P][k * S::NC + S::NC - 1], qa1, s);
-            w[d] = s;
+            for (int v = 0; v < 5; ++v) Qv[v] = QA[((size_t)mo * NEQ + v) * S::NF + q];
+        } else {
+#pragma unroll
+            for (int v = 0; v < 5; ++v) {
+                double t = 0.0;
+#pragma unroll
+                for (int k = 0; k < S::NCG; ++k) t = fma(Bj[k], sQ[v * S::NP + gidx<P>(D, k, a, b)], t);
+                Qv[v] = t;
+                if (v == 1 || v == 3) __builtin_amdgcn_sched_barrier(0);
+            }
         }
-        const int pq = pt * m.ms;
+        const Prim pr = primitives(ph, Qv);
+        __builtin_amdgcn_sched_barrier(0);
+        double Lg[3][3], Tg[3];
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-            const double g0 = gg[(0 * 3 + c) * npm + pq], g1 = gg[(1 * 3 + c) * npm + pq], g2 = gg[(2 * 3 + c) * npm + pq];
-            Gr[((size_t)e * 15 + c * 5 + v) * S::NP + pt] = g0 * w[0] + g1 * w[1] + g2 * w[2];
+            double Gc[5];
+#pragma unroll
+            for (int v = 0; v < 5; ++v) {
+                const double *gl = sG + (c * 5 + v) * S::NP;
+                double t = 0.0;
+#pragma unroll
+                for (int k = 0; k < S::NCG; ++k) t = fma(Bj[k], gl[gidx<P>(D, k, a, b)], t);
+                Gc[v] = t;
+                if (v == 1 || v == 3) __builtin_amdgcn_sched_barrier(0);
+            }
+            double o[4];
+            prim_gradient(ph, pr, Gc, o);
+            Lg[0][c] = o[0];
+            Lg[1][c] = o[1];
+            Lg[2][c] = o[2];
+            Tg[c] = o[3];
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        double amu = 1.0;
+        if (muh) amu = amu + muh[((size_t)e * 3 + D) * S::NS + s];
+        const int sq = s * m.ms;
+        const double g0 = gmd[0 * nsm + sq], g1 = gmd[1 * nsm + sq], g2 = gmd[2 * nsm + sq];
+        double rv[5];
+        viscous_from_primitive(ph, pr, Lg, Tg, amu, g0, g1, g2, rv);
+        if (face) {
+            const double nrm = m.norm[(size_t)mo * nfm + q * m.ms];
+#pragma unroll
+            for (int v = 0; v < 5; ++v) FV[(((size_t)e * 6 + fc - 1) * NEQ + v) * S::NF + q] = rv[v] / nrm;
+        } else {
+            double fi[5];
+            euler_contravariant(ph, Qv, g0, g1, g2, fi);
+#pragma unroll
+            for (int v = 0; v < 5; ++v) sF[v * S::NS + s] = fi[v] - rv[v] * inv_re;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int pt = tid + r * VBLOCK;
+        if (pt < S::NP) {
+            const int n = pt % S::NCG, mi = (pt / S::NCG) % S::NCG, l = pt / (S::NCG * S::NCG);
+            const int k = D == 0 ? n : (D == 1 ? mi : l);
+            const int a = D == 0 ? mi : n;
+            const int b = D == 2 ? mi : l;
+            double Dk[S::NC];
+#pragma unroll
+            for (int j = 1; j < S::NC - 1; ++j) Dk[j] = c_D[P][k * S::NC + j];
+#pragma unroll
+            for (int v = 0; v < 5; ++v) {
+                double t = 0.0;
+#pragma unroll
+                for (int j = 1; j < S::NC - 1; ++j) t = fma(Dk[j], sF[v * S::NS + sidx<P>(D, j, a, b)], t);
+                acc[r][v] -= t;
+            }
         }
     }
 }
 
+// Fused volume kernel: Gauss-point gradients into LDS (Gauss_Deriv,
+// SolnDerivsc.f90:14-83), then for each staggered grid the interpolated
+// state, prolonged gradients, contravariant Euler + viscous fluxes, the
+// viscous face traces r/norm and the interior part of the divergence.
 template <int P>
-__global__ __launch_bounds__(BLOCK) void k_volume(const double *__restrict__ JQ, DevMesh m, DevPhys ph,
-                                                   const double *__restrict__ QA, const double *__restrict__ Gr,
-                                                   const double *__restrict__ muh, double *__restrict__ FV,
-                                                   double *__restrict__ Qtp)
+__global__ __launch_bounds__(VBLOCK) __attribute__((amdgpu_waves_per_eu(1, 1))) void k_volume(const double *__restrict__ JQ, DevMesh m, DevPhys ph,
+                                                    const double *__restrict__ QA, const double *__restrict__ muh,
+                                                    double *__restrict__ FV, double *__restrict__ Qtp)
 {
     using S = Sz<P>;
-    __shared__ double sQ[NEQ * S::NP];
-    __shared__ double sF[NEQ * S::NS];
+    constexpr int R = (S::NP + VBLOCK - 1) / VBLOCK;
+    __shared__ double sQ[NEQ * S::NP];        // physical Q at Gauss points
+    __shared__ double sG[15 * S::NP];         // dQ_v/dx_c at Gauss points, (c*5+v)
+    __shared__ double sF[NEQ * S::NS];        // interior total flux of the current grid
     const int e = blockIdx.x;
-    load_physical_state<P>(JQ, m, e, sQ);
+    const int tid = threadIdx.x;
+    {
+        const double *q = JQ + (size_t)e * NEQ * S::NP;
+        const double *jac = m.jacob + (size_t)e * (m.ms ? S::NP : 1);
+        for (int i = tid; i < NEQ * S::NP; i += VBLOCK) sQ[i] = q[i] * jac[(i % S::NP) * m.ms];
+    }
     const int *e2m = m.e2m + (size_t)e * 6;
-    const int nsm = m.ms ? S::NS : 1;
-    const int nfm = m.ms ? S::NF : 1;
-    const double *gre = Gr + (size_t)e * 15 * S::NP;
-    const double inv_re = 1.0 / ph.re;
-    double acc[S::R][5];
+    int mort[6];
 #pragma unroll
-    for (int r = 0; r < S::R; ++r)
-#pragma unroll
-        for (int v = 0; v < 5; ++v) acc[r][v] = 0.0;
+    for (int f = 0; f < 6; ++f) mort[f] = e2m[f] >> 1;
+    const int npm = m.ms ? S::NP : 1, nsm = m.ms ? S::NS : 1, nfm = m.ms ? S::NF : 1;
+    __syncthreads();
 
-    for (int d = 0; d < 3; ++d) {
-        const double *gmd = d == 0 ? m.gmet0 : (d == 1 ? m.gmet1 : m.gmet2);
-        gmd += (size_t)e * 3 * nsm;
-        __syncthreads();
-        for (int s = threadIdx.x; s < S::NS; s += BLOCK) {
-            int j, a, b;
-            if (d == 0) { j = s % S::NC; a = (s / S::NC) % S::NCG; b = s / (S::NC * S::NCG); }
-            else if (d == 1) { a = s % S::NCG; j = (s / S::NCG) % S::NC; b = s / (S::NCG * S::NC); }
-            else { a = s % S::NCG; b = (s / S::NCG) % S::NCG; j = s / (S::NCG * S::NCG); }
-            const bool face = (j == 0 || j == S::NC - 1);
-            int mort = 0, fc = 0;
-            const int q = a + S::NCG * b;
-            double Qv[5];
-            if (face) {
-                fc = face_of<P>(d, j);
-                mort = e2m[fc - 1] >> 1;
+    // ---- gradients: w_d = (D_int B) Q + D(:,0) QA_lo + D(:,nc-1) QA_hi, then metric combine
+    {
+        const double *gg = m.gmetg + (size_t)e * 9 * npm;
+#pragma unroll 1
+        for (int pt = tid; pt < S::NP; pt += VBLOCK) {
+            const int n = pt % S::NCG, mi = (pt / S::NCG) % S::NCG, l = pt / (S::NCG * S::NCG);
+            const int pq = pt * m.ms;
+            double out[3][5];
 #pragma unroll
-                for (int v = 0; v < 5; ++v) Qv[v] = QA[((size_t)mort * NEQ + v) * S::NF + q];
-            } else {
-#pragma unroll
-                for (int v = 0; v < 5; ++v) {
-                    double t = 0.0;
-#pragma unroll
-                    for (int k = 0; k < S::NCG; ++k) t = fma(c_B[P][j * S::NCG + k], sQ[v * S::NP + gidx<P>(d, k, a, b)], t);
-                    Qv[v] = t;
-                }
-            }
-            double Gx[3][5];
-#pragma unroll
-            for (int c = 0; c < 3; ++c)
-#pragma unroll
-                for (int v = 0; v < 5; ++v) {
-                    const double *gl = gre + (c * 5 + v) * S::NP;
-                    double t = 0.0;
-#pragma unroll
-                    for (int k = 0; k < S::NCG; ++k) t = fma(c_B[P][j * S::NCG + k], gl[gidx<P>(d, k, a, b)], t);
-                    Gx[c][v] = t;
-                }
-            double amu = 1.0;
-            if (muh) amu = amu + muh[((size_t)e * 3 + d) * S::NS + s];
-            const int sq = s * m.ms;
-            const double g0 = gmd[0 * nsm + sq], g1 = gmd[1 * nsm + sq], g2 = gmd[2 * nsm + sq];
-            double rv[5];
-            viscous_contravariant(ph, Qv, Gx[0], Gx[1], Gx[2], amu, g0, g1, g2, rv);
-            if (face) {
-                const double nrm = m.norm[(size_t)mort * nfm + q * m.ms];
-#pragma unroll
-                for (int v = 0; v < 5; ++v) FV[(((size_t)e * 6 + fc - 1) * NEQ + v) * S::NF + q] = rv[v] / nrm;
-            } else {
-                double fi[5];
-                euler_contravariant(ph, Qv, g0, g1, g2, fi);
-#pragma unroll
-                for (int v = 0; v < 5; ++v) sF[v * S::NS + s] = fi[v] - rv[v] * inv_re;
-            }
-        }
-        __syncthreads();
-#pragma unroll
-        for (int r = 0; r < S::R; ++r) {
-            const int pt = threadIdx.x + r * BLOCK;
-            if (pt < S::NP) {
-                const int n = pt % S::NCG, mi = (pt / S::NCG) % S::NCG, l = pt / (S::NCG * S::NCG);
+            for (int d = 0; d < 3; ++d) {
                 const int k = d == 0 ? n : (d == 1 ? mi : l);
                 const int a = d == 0 ? mi : n;
                 const int b = d == 2 ? mi : l;
+                const int q = a + S::NCG * b;
+                double Mk[S::NCG];
+#pragma unroll
+                for (int k2 = 0; k2 < S::NCG; ++k2) Mk[k2] = c_M[P][k * S::NCG + k2];
+                const double d0 = c_D[P][k * S::NC], d1 = c_D[P][k * S::NC + S::NC - 1];
+                const double *qa0 = QA + (size_t)mort[face_of<P>(d, 0) - 1] * NEQ * S::NF + q;
+                const double *qa1 = QA + (size_t)mort[face_of<P>(d, 1) - 1] * NEQ * S::NF + q;
+                double gd[3];
+#pragma unroll
+                for (int c = 0; c < 3; ++c) gd[c] = gg[(d * 3 + c) * npm + pq];
 #pragma unroll
                 for (int v = 0; v < 5; ++v) {
                     double t = 0.0;
 #pragma unroll
-                    for (int j = 1; j < S::NC - 1; ++j) t = fma(c_D[P][k * S::NC + j], sF[v * S::NS + sidx<P>(d, j, a, b)], t);
-                    acc[r][v] -= t;
+                    for (int k2 = 0; k2 < S::NCG; ++k2) t = fma(Mk[k2], sQ[v * S::NP + gidx<P>(d, k2, a, b)], t);
+                    t = fma(d0, qa0[v * S::NF], t);
+                    t = fma(d1, qa1[v * S::NF], t);
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) out[c][v] = d == 0 ? gd[c] * t : fma(gd[c], t, out[c][v]);
                 }
+                __builtin_amdgcn_sched_barrier(0);
             }
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+#pragma unroll
+                for (int v = 0; v < 5; ++v) sG[(c * 5 + v) * S::NP + pt] = out[c][v];
         }
     }
+
+    double acc[R][5];
 #pragma unroll
-    for (int r = 0; r < S::R; ++r) {
-        const int pt = threadIdx.x + r * BLOCK;
+    for (int r = 0; r < R; ++r)
+#pragma unroll
+        for (int v = 0; v < 5; ++v) acc[r][v] = 0.0;
+    volume_grid<P, 0>(m, ph, QA, muh, FV, sQ, sG, sF, e, mort, acc);
+    volume_grid<P, 1>(m, ph, QA, muh, FV, sQ, sG, sF, e, mort, acc);
+    volume_grid<P, 2>(m, ph, QA, muh, FV, sQ, sG, sF, e, mort, acc);
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int pt = tid + r * VBLOCK;
         if (pt < S::NP)
 #pragma unroll
             for (int v = 0; v < 5; ++v) Qtp[((size_t)e * NEQ + v) * S::NP + pt] = acc[r][v];
@@ -524,9 +625,9 @@ __global__ __launch_bounds__(BLOCK) void k_dtcalc(const double *__restrict__ JQ,
 // ====================================================================== host
 namespace {
 
-enum KernelId { KID_TRACES = 0, KID_FACE_INV, KID_GRAD, KID_VOLUME, KID_FACE_VISC, KID_UPDATE, KID_DTCALC, KID_COUNT };
-const char *const kKernelNames[KID_COUNT] = {"k_traces", "k_face_inv", "k_grad", "k_volume",
-                                             "k_face_visc", "k_update", "k_dtcalc"};
+enum KernelId { KID_TRACES = 0, KID_FACE_INV, KID_VOLUME, KID_FACE_VISC, KID_UPDATE, KID_DTCALC, KID_COUNT };
+const char *const kKernelNames[KID_COUNT] = {"k_traces", "k_face_inv", "k_volume", "k_face_visc",
+                                             "k_update", "k_dtcalc"};
 
 template <typename T>
 struct DevBuf {
@@ -547,7 +648,7 @@ struct dsem_ctx {
     std::vector<double> h_xg, h_jacob;   // Gauss points and 1/J on the host
     // device buffers
     double *JQ = nullptr, *G = nullptr, *Qtp = nullptr, *Qt = nullptr;
-    double *Gr = nullptr, *T = nullptr, *FV = nullptr, *QA = nullptr, *FI = nullptr, *FT = nullptr;
+    double *T = nullptr, *FV = nullptr, *QA = nullptr, *FI = nullptr, *FT = nullptr;
     double *jacob = nullptr, *gmetg = nullptr, *gmet[3] = {nullptr, nullptr, nullptr};
     double *nhat = nullptr, *norm = nullptr, *muh = nullptr;
     int *e2m = nullptr, *m_elem = nullptr, *m_face = nullptr, *m_sign = nullptr, *m_nsign = nullptr;
@@ -648,9 +749,7 @@ int run_rhs(dsem_ctx *c, double *JQ, const StageArgs &st, double *Qt_out)
     hipStream_t s = c->stream;
     TRY(launch(c, KID_TRACES, [&] { k_traces<P><<<K, BLOCK, 0, s>>>(JQ, m, c->T); }));
     TRY(launch(c, KID_FACE_INV, [&] { k_face_inv<P><<<face_blocks, BLOCK, 0, s>>>(c->T, m, ph, c->FI, c->QA); }));
-    TRY(launch(c, KID_GRAD, [&] { k_grad<P><<<K, BLOCK, 0, s>>>(JQ, m, c->QA, c->Gr); }));
-    TRY(launch(c, KID_VOLUME,
-               [&] { k_volume<P><<<K, BLOCK, 0, s>>>(JQ, m, ph, c->QA, c->Gr, muh, c->FV, c->Qtp); }));
+    TRY(launch(c, KID_VOLUME, [&] { k_volume<P><<<K, VBLOCK, 0, s>>>(JQ, m, ph, c->QA, muh, c->FV, c->Qtp); }));
     TRY(launch(c, KID_FACE_VISC, [&] { k_face_visc<P><<<face_blocks, BLOCK, 0, s>>>(c->FV, c->FI, m, ph, c->FT); }));
     TRY(launch(c, KID_UPDATE, [&] { k_update<P><<<K, BLOCK, 0, s>>>(c->Qtp, c->FT, m, st, JQ, c->G, Qt_out); }));
     return DSEM_OK;
@@ -788,7 +887,6 @@ int create_impl(const dsem_mesh_desc *mesh, const dsem_params *prm, int device, 
     TRY(dev_alloc(c, &c->G, nst));
     TRY(dev_alloc(c, &c->Qtp, nst));
     TRY(dev_alloc(c, &c->Qt, nst));
-    TRY(dev_alloc(c, &c->Gr, (size_t)c->K * 15 * c->np));
     TRY(dev_alloc(c, &c->T, (size_t)c->K * 6 * NEQ * c->nf));
     TRY(dev_alloc(c, &c->FV, (size_t)c->K * 6 * NEQ * c->nf));
     TRY(dev_alloc(c, &c->QA, (size_t)c->M * NEQ * c->nf));
