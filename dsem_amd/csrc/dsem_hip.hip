@@ -50,7 +50,7 @@ namespace {
 
 constexpr int NEQ = 5;
 constexpr int BLOCK = 256;
-constexpr int VBLOCK = 256;   // fused volume kernel: one element per workgroup, one wave per SIMD
+constexpr int VBLOCK = 512;   // fused volume kernel: one element per workgroup, 8 waves
 
 // operators for every order p = 1..8, indexed [p]
 __constant__ double c_D[9][90];   // D[k*nc + j]
@@ -75,6 +75,11 @@ struct DevMesh {
 
 struct DevPhys {
     double gamma, re, pr, mach;
+    double gm1;        // gamma - 1
+    double tfac;       // mach^2 gamma          (T = tfac p / rho)
+    double hfac1;      // 1 / (pr mach^2 (gamma-1))
+    double kap;        // 0.2 / (gamma-1)       (kappa = 1 + (mu-1) kap)
+    double inv_re;
 };
 
 // Lobatto index j of grid d at line (a,b) -> staggered index
@@ -102,99 +107,57 @@ __device__ __forceinline__ int face_of(int d, int j)
 }
 
 // ------------------------------------------------------------ pointwise physics
-// Euler fluxes of PhysLib.f90:17-124, contracted with one metric row.
-__device__ __forceinline__ void euler_contravariant(const DevPhys &ph, const double *Q, double g0, double g1, double g2,
-                                                    double *out)
-{
-    const double rho = Q[0], rhou = Q[1], rhov = Q[2], rhow = Q[3], rhoe = Q[4];
-    const double u = rhou / rho, v = rhov / rho, w = rhow / rho;
-    const double p = (ph.gamma - 1.0) * (rhoe - 0.5 * rho * (u * u + v * v + w * w));
-    const double f[5] = {rhou, p + rhou * u, rhou * v, rhou * w, u * (rhoe + p)};
-    const double g[5] = {rhov, rhou * v, p + rhov * v, rhow * v, v * (rhoe + p)};
-    const double h[5] = {rhow, rhou * w, rhov * w, p + rhow * w, w * (rhoe + p)};
-#pragma unroll
-    for (int v2 = 0; v2 < 5; ++v2) out[v2] = g0 * f[v2] + g1 * g[v2] + g2 * h[v2];
-}
-
-// x/y/z_vis_flux of PhysLib.f90:419-583 contracted with one metric row.
-__device__ __forceinline__ void viscous_contravariant(const DevPhys &ph, const double *Q, const double *Qx,
-                                                      const double *Qy, const double *Qz, double amu, double g0,
-                                                      double g1, double g2, double *r)
-{
-    const double gamma = ph.gamma, mach = ph.mach;
-    const double rho = Q[0];
-    const double u = Q[1] / rho, v = Q[2] / rho, w = Q[3] / rho;
-    const double q2 = u * u + v * v + w * w;
-    const double p = (gamma - 1.0) * (Q[4] - 0.5 * rho * q2);
-    const double ux = (Qx[1] - u * Qx[0]) / rho, uy = (Qy[1] - u * Qy[0]) / rho, uz = (Qz[1] - u * Qz[0]) / rho;
-    const double vx = (Qx[2] - v * Qx[0]) / rho, vy = (Qy[2] - v * Qy[0]) / rho, vz = (Qz[2] - v * Qz[0]) / rho;
-    const double wx = (Qx[3] - w * Qx[0]) / rho, wy = (Qy[3] - w * Qy[0]) / rho, wz = (Qz[3] - w * Qz[0]) / rho;
-    const double akappa = 1.0 + (amu - 1.0) * 2.0e-1 / (gamma - 1.0);
-    const double div = ux + vy + wz;
-    const double tfac = mach * mach * gamma;
-    const double hfac = akappa / (ph.pr * mach * mach * (gamma - 1.0));
-    const double px = (gamma - 1.0) * (Qx[4] - 0.5 * Qx[0] * q2 - rho * (u * ux + v * vx + w * wx));
-    const double py = (gamma - 1.0) * (Qy[4] - 0.5 * Qy[0] * q2 - rho * (u * uy + v * vy + w * wy));
-    const double pz = (gamma - 1.0) * (Qz[4] - 0.5 * Qz[0] * q2 - rho * (u * uz + v * vz + w * wz));
-    const double tx = (px - p * Qx[0] / rho) * tfac / rho;
-    const double ty = (py - p * Qy[0] / rho) * tfac / rho;
-    const double tz = (pz - p * Qz[0] / rho) * tfac / rho;
-    const double txx = 2.0 * amu * (ux - div / 3.0), tyy = 2.0 * amu * (vy - div / 3.0), tzz = 2.0 * amu * (wz - div / 3.0);
-    const double txy = amu * (uy + vx), txz = amu * (wx + uz), tyz = amu * (wy + vz);
-    const double f4 = u * txx + v * txy + w * txz + hfac * tx;
-    const double g4 = v * tyy + u * txy + w * tyz + hfac * ty;
-    const double h4 = v * tyz + u * txz + w * tzz + hfac * tz;
-    r[0] = 0.0;
-    r[1] = g0 * txx + g1 * txy + g2 * txz;
-    r[2] = g0 * txy + g1 * tyy + g2 * tyz;
-    r[3] = g0 * txz + g1 * tyz + g2 * tzz;
-    r[4] = g0 * f4 + g1 * g4 + g2 * h4;
-}
-
-// The same viscous flux split in two steps so that the conservative
-// gradients of one direction can be consumed as soon as they are formed:
-// prim_gradient turns dQ/dx_c into (du/dx_c, dv/dx_c, dw/dx_c, dT/dx_c)
-// with the expressions of x/y/z_vis_flux (PhysLib.f90:448-462), and
-// viscous_from_primitive forms the stress/heat flux (PhysLib.f90:464-470).
+// Pointwise physics.  Euler fluxes: fflux/gflux/hflux (PhysLib.f90:17-124).
+// Viscous fluxes x/y/z_vis_flux (PhysLib.f90:419-583), split in two steps so
+// that the conservative gradients of one direction are consumed as soon as
+// they are formed: prim_gradient turns dQ/dx_c into (du/dx_c, dv/dx_c,
+// dw/dx_c, dT/dx_c) (PhysLib.f90:448-462) and viscous_from_primitive forms
+// the Stokes stress and Fourier heat flux (PhysLib.f90:464-470).  Both are
+// contracted with one metric row J grad xi_d as Int_fluxes/Vis_Fluxes do.
 struct Prim {
-    double rho, u, v, w, q2, p;
+    double rho, rinv, u, v, w, q2, p;
 };
 
+// Divisions by rho are multiplications by 1/rho here (one rounding apart
+// from the reference's quotients).
 __device__ __forceinline__ Prim primitives(const DevPhys &ph, const double *Q)
 {
     Prim s;
     s.rho = Q[0];
-    s.u = Q[1] / s.rho;
-    s.v = Q[2] / s.rho;
-    s.w = Q[3] / s.rho;
+    s.rinv = 1.0 / s.rho;
+    s.u = Q[1] * s.rinv;
+    s.v = Q[2] * s.rinv;
+    s.w = Q[3] * s.rinv;
     s.q2 = s.u * s.u + s.v * s.v + s.w * s.w;
-    s.p = (ph.gamma - 1.0) * (Q[4] - 0.5 * s.rho * s.q2);
+    s.p = ph.gm1 * (Q[4] - 0.5 * s.rho * s.q2);
     return s;
 }
 
 __device__ __forceinline__ void prim_gradient(const DevPhys &ph, const Prim &s, const double *Qc, double *out)
 {
-    const double uc = (Qc[1] - s.u * Qc[0]) / s.rho;
-    const double vc = (Qc[2] - s.v * Qc[0]) / s.rho;
-    const double wc = (Qc[3] - s.w * Qc[0]) / s.rho;
-    const double pc = (ph.gamma - 1.0) * (Qc[4] - 0.5 * Qc[0] * s.q2 - s.rho * (s.u * uc + s.v * vc + s.w * wc));
+    const double uc = (Qc[1] - s.u * Qc[0]) * s.rinv;
+    const double vc = (Qc[2] - s.v * Qc[0]) * s.rinv;
+    const double wc = (Qc[3] - s.w * Qc[0]) * s.rinv;
+    const double pc = ph.gm1 * (Qc[4] - 0.5 * Qc[0] * s.q2 - s.rho * (s.u * uc + s.v * vc + s.w * wc));
     out[0] = uc;
     out[1] = vc;
     out[2] = wc;
-    out[3] = (pc - s.p * Qc[0] / s.rho) * (ph.mach * ph.mach * ph.gamma) / s.rho;
+    out[3] = (pc - s.p * Qc[0] * s.rinv) * ph.tfac * s.rinv;
 }
 
-// L[i][c] = d u_i / d x_c (i = u,v,w), T[c] = dT/dx_c
-__device__ __forceinline__ void viscous_from_primitive(const DevPhys &ph, const Prim &s, const double L[3][3],
-                                                       const double *T, double amu, double g0, double g1,
-                                                       double g2, double *r)
+// L[i][c] = d u_i / d x_c (i = u,v,w), T[c] = dT/dx_c.  Kept out of line:
+// when inlined, the compiler interleaves it with the caller's prolongation
+// loads and runs out of registers; the call boundary (arguments through the
+// stack) keeps the kernel at about 150 VGPRs.
+__device__ __attribute__((noinline)) void viscous_from_primitive(const DevPhys &ph, const Prim &s, const double L[3][3],
+                                                                 const double *T, double amu, double g0, double g1,
+                                                                 double g2, double *r)
 {
-    const double gamma = ph.gamma, mach = ph.mach;
-    const double akappa = 1.0 + (amu - 1.0) * 2.0e-1 / (gamma - 1.0);
-    const double hfac = akappa / (ph.pr * mach * mach * (gamma - 1.0));
+    const double hfac = (1.0 + (amu - 1.0) * ph.kap) * ph.hfac1;
+    const double third = 1.0 / 3.0;
     const double div = L[0][0] + L[1][1] + L[2][2];
-    const double txx = 2.0 * amu * (L[0][0] - div / 3.0), tyy = 2.0 * amu * (L[1][1] - div / 3.0);
-    const double tzz = 2.0 * amu * (L[2][2] - div / 3.0);
+    const double txx = 2.0 * amu * (L[0][0] - div * third), tyy = 2.0 * amu * (L[1][1] - div * third);
+    const double tzz = 2.0 * amu * (L[2][2] - div * third);
     const double txy = amu * (L[0][1] + L[1][0]), txz = amu * (L[2][0] + L[0][2]), tyz = amu * (L[2][1] + L[1][2]);
     const double f4 = s.u * txx + s.v * txy + s.w * txz + hfac * T[0];
     const double g4 = s.v * tyy + s.u * txy + s.w * tyz + hfac * T[1];
@@ -204,6 +167,18 @@ __device__ __forceinline__ void viscous_from_primitive(const DevPhys &ph, const 
     r[2] = g0 * txy + g1 * tyy + g2 * tyz;
     r[3] = g0 * txz + g1 * tyz + g2 * tzz;
     r[4] = g0 * f4 + g1 * g4 + g2 * h4;
+}
+
+// Euler flux contracted with one metric row, from primitives.
+__device__ __forceinline__ void euler_from_primitive(const Prim &s, const double *Q, double g0, double g1, double g2,
+                                                     double *out)
+{
+    const double qn = g0 * s.u + g1 * s.v + g2 * s.w;   // contravariant velocity
+    out[0] = g0 * Q[1] + g1 * Q[2] + g2 * Q[3];
+    out[1] = g0 * (s.p + Q[1] * s.u) + g1 * (Q[1] * s.v) + g2 * (Q[1] * s.w);
+    out[2] = g0 * (Q[1] * s.v) + g1 * (s.p + Q[2] * s.v) + g2 * (Q[3] * s.v);
+    out[3] = g0 * (Q[1] * s.w) + g1 * (Q[2] * s.w) + g2 * (s.p + Q[3] * s.w);
+    out[4] = qn * (Q[4] + s.p);
 }
 
 // Roe flux with entropy fix (PhysLib.f90:231-320)
@@ -334,7 +309,6 @@ __device__ __forceinline__ void volume_grid(const DevMesh &m, const DevPhys &ph,
     const int tid = threadIdx.x;
     const int nsm = m.ms ? S::NS : 1, nfm = m.ms ? S::NF : 1;
     const double *gmd = (D == 0 ? m.gmet0 : (D == 1 ? m.gmet1 : m.gmet2)) + (size_t)e * 3 * nsm;
-    const double inv_re = 1.0 / ph.re;
     __syncthreads();   // sG complete / previous divergence pass done with sF
 #pragma unroll 1
     for (int s = tid; s < S::NS; s += VBLOCK) {
@@ -398,9 +372,9 @@ __device__ __forceinline__ void volume_grid(const DevMesh &m, const DevPhys &ph,
             for (int v = 0; v < 5; ++v) FV[(((size_t)e * 6 + fc - 1) * NEQ + v) * S::NF + q] = rv[v] / nrm;
         } else {
             double fi[5];
-            euler_contravariant(ph, Qv, g0, g1, g2, fi);
+            euler_from_primitive(pr, Qv, g0, g1, g2, fi);
 #pragma unroll
-            for (int v = 0; v < 5; ++v) sF[v * S::NS + s] = fi[v] - rv[v] * inv_re;
+            for (int v = 0; v < 5; ++v) sF[v * S::NS + s] = fi[v] - rv[v] * ph.inv_re;
         }
     }
     __syncthreads();
@@ -431,7 +405,7 @@ __device__ __forceinline__ void volume_grid(const DevMesh &m, const DevPhys &ph,
 // state, prolonged gradients, contravariant Euler + viscous fluxes, the
 // viscous face traces r/norm and the interior part of the divergence.
 template <int P>
-__global__ __launch_bounds__(VBLOCK) __attribute__((amdgpu_waves_per_eu(1, 1))) void k_volume(const double *__restrict__ JQ, DevMesh m, DevPhys ph,
+__global__ __launch_bounds__(VBLOCK) void k_volume(const double *__restrict__ JQ, DevMesh m, DevPhys ph,
                                                     const double *__restrict__ QA, const double *__restrict__ muh,
                                                     double *__restrict__ FV, double *__restrict__ Qtp)
 {
@@ -451,7 +425,7 @@ __global__ __launch_bounds__(VBLOCK) __attribute__((amdgpu_waves_per_eu(1, 1))) 
     int mort[6];
 #pragma unroll
     for (int f = 0; f < 6; ++f) mort[f] = e2m[f] >> 1;
-    const int npm = m.ms ? S::NP : 1, nsm = m.ms ? S::NS : 1, nfm = m.ms ? S::NF : 1;
+    const int npm = m.ms ? S::NP : 1;
     __syncthreads();
 
     // ---- gradients: w_d = (D_int B) Q + D(:,0) QA_lo + D(:,nc-1) QA_hi, then metric combine
@@ -544,14 +518,29 @@ struct StageArgs {
     unsigned int *nonfinite;       // set when any Qt is NaN/Inf (nullable)
 };
 
-template <int P>
+// Face part of the divergence, Qt, and (rk) the register update
+// g = a g + Qt, JQ += c dt g (updatea.f90:213-227).  With TRACES the updated
+// state is staged in LDS and the face traces of the next stage are computed
+// here, so the state is not read again by k_traces.
+template <int P, bool TRACES>
 __global__ __launch_bounds__(BLOCK) void k_update(const double *__restrict__ Qtp, const double *__restrict__ FT, DevMesh m,
                                                    StageArgs st, double *__restrict__ JQ, double *__restrict__ G,
-                                                   double *__restrict__ Qt)
+                                                   double *__restrict__ Qt, double *__restrict__ T)
 {
     using S = Sz<P>;
+    __shared__ double sFT[6 * NEQ * S::NF];
+    __shared__ double sQ[TRACES ? NEQ * S::NP : 1];
     const int e = blockIdx.x;
     const int *e2m = m.e2m + (size_t)e * 6;
+    for (int i = threadIdx.x; i < 6 * NEQ * S::NF; i += BLOCK) {
+        const int f = i / (NEQ * S::NF), r = i % (NEQ * S::NF);
+        const int c = e2m[f];
+        double val = FT[(size_t)(c >> 1) * NEQ * S::NF + r];
+        if ((c & 1) && m.m_sign[c >> 1] == -1) val = -val;
+        sFT[i] = val;
+    }
+    __syncthreads();
+    const double *jac = m.jacob + (size_t)e * (m.ms ? S::NP : 1);
     double mx = 0.0;
     bool bad = false;
     for (int i = threadIdx.x; i < NEQ * S::NP; i += BLOCK) {
@@ -564,19 +553,17 @@ __global__ __launch_bounds__(BLOCK) void k_update(const double *__restrict__ Qtp
             const int a = d == 0 ? mi : n;
             const int b = d == 2 ? mi : l;
             const int q = a + S::NCG * b;
-            const int f0 = face_of<P>(d, 0), f1 = face_of<P>(d, 1);
-            const int c0 = e2m[f0 - 1], c1 = e2m[f1 - 1];
-            double F0 = FT[((size_t)(c0 >> 1) * NEQ + v) * S::NF + q];
-            double F1 = FT[((size_t)(c1 >> 1) * NEQ + v) * S::NF + q];
-            if ((c0 & 1) && m.m_sign[c0 >> 1] == -1) F0 = -F0;
-            if ((c1 & 1) && m.m_sign[c1 >> 1] == -1) F1 = -F1;
+            const double F0 = sFT[((face_of<P>(d, 0) - 1) * NEQ + v) * S::NF + q];
+            const double F1 = sFT[((face_of<P>(d, 1) - 1) * NEQ + v) * S::NF + q];
             qt = qt - fma(c_D[P][k * S::NC + 0], F0, c_D[P][k * S::NC + S::NC - 1] * F1);
         }
         const size_t gi = (size_t)e * NEQ * S::NP + i;
         if (st.rk) {
             const double g = st.first ? qt : fma(st.a, G[gi], qt);
             G[gi] = g;
-            JQ[gi] = fma(st.cdt, g, JQ[gi]);
+            const double jq = fma(st.cdt, g, JQ[gi]);
+            JQ[gi] = jq;
+            if (TRACES) sQ[i] = jq * jac[pt * m.ms];
         } else {
             Qt[gi] = qt;
         }
@@ -585,6 +572,20 @@ __global__ __launch_bounds__(BLOCK) void k_update(const double *__restrict__ Qtp
     }
     if (bad && st.nonfinite) atomicOr(st.nonfinite, 1u);
     if (st.resid && e == m.K - 1) atomicMax(st.resid, (unsigned long long)__double_as_longlong(mx));
+    if (TRACES) {
+        __syncthreads();
+        double *t = T + (size_t)e * 6 * NEQ * S::NF;
+        for (int i = threadIdx.x; i < 6 * NEQ * S::NF; i += BLOCK) {
+            const int qf = i % S::NF, v = (i / S::NF) % NEQ, f = i / (NEQ * S::NF) + 1;
+            const int a = qf % S::NCG, b = qf / S::NCG;
+            const int d = (f == 4 || f == 6) ? 0 : ((f == 1 || f == 2) ? 1 : 2);
+            const int j = (f == 2 || f == 4 || f == 5) ? S::NC - 1 : 0;
+            double s2 = 0.0;
+#pragma unroll
+            for (int k = 0; k < S::NCG; ++k) s2 = fma(c_B[P][j * S::NCG + k], sQ[v * S::NP + gidx<P>(d, k, a, b)], s2);
+            t[i] = s2;
+        }
+    }
 }
 
 // dtcalc (dtcalc.f90:9-58): per point 1/dti, reduced to the minimum
@@ -708,7 +709,18 @@ DevMesh dev_mesh(const dsem_ctx *c)
 
 DevPhys dev_phys(const dsem_ctx *c)
 {
-    return DevPhys{c->params.gamma, c->params.re, c->params.pr, c->params.mach};
+    const dsem_params &p = c->params;
+    DevPhys d;
+    d.gamma = p.gamma;
+    d.re = p.re;
+    d.pr = p.pr;
+    d.mach = p.mach;
+    d.gm1 = p.gamma - 1.0;
+    d.tfac = p.mach * p.mach * p.gamma;
+    d.hfac1 = 1.0 / (p.pr * p.mach * p.mach * (p.gamma - 1.0));
+    d.kap = 2.0e-1 / (p.gamma - 1.0);
+    d.inv_re = 1.0 / p.re;
+    return d;
 }
 
 // Launch helper: brackets the launch with events when timing is on.
@@ -737,8 +749,11 @@ int launch(dsem_ctx *c, int kid, F &&fn)
 
 // One RHS evaluation on state JQ; the final kernel either writes Qt or
 // performs an RK register update (st.rk).
+// traces_ready: T already holds the face traces of JQ (computed by the
+// previous stage's update); make_traces: this stage's update also computes
+// the traces of the updated state for the next stage.
 template <int P>
-int run_rhs(dsem_ctx *c, double *JQ, const StageArgs &st, double *Qt_out)
+int run_rhs(dsem_ctx *c, double *JQ, const StageArgs &st, double *Qt_out, bool traces_ready, bool make_traces)
 {
     using S = Sz<P>;
     const DevMesh m = dev_mesh(c);
@@ -747,11 +762,14 @@ int run_rhs(dsem_ctx *c, double *JQ, const StageArgs &st, double *Qt_out)
     const int face_blocks = (c->M * S::NF + BLOCK - 1) / BLOCK;
     const double *muh = c->params.shock ? c->muh : nullptr;
     hipStream_t s = c->stream;
-    TRY(launch(c, KID_TRACES, [&] { k_traces<P><<<K, BLOCK, 0, s>>>(JQ, m, c->T); }));
+    if (!traces_ready) TRY(launch(c, KID_TRACES, [&] { k_traces<P><<<K, BLOCK, 0, s>>>(JQ, m, c->T); }));
     TRY(launch(c, KID_FACE_INV, [&] { k_face_inv<P><<<face_blocks, BLOCK, 0, s>>>(c->T, m, ph, c->FI, c->QA); }));
     TRY(launch(c, KID_VOLUME, [&] { k_volume<P><<<K, VBLOCK, 0, s>>>(JQ, m, ph, c->QA, muh, c->FV, c->Qtp); }));
     TRY(launch(c, KID_FACE_VISC, [&] { k_face_visc<P><<<face_blocks, BLOCK, 0, s>>>(c->FV, c->FI, m, ph, c->FT); }));
-    TRY(launch(c, KID_UPDATE, [&] { k_update<P><<<K, BLOCK, 0, s>>>(c->Qtp, c->FT, m, st, JQ, c->G, Qt_out); }));
+    if (make_traces)
+        TRY(launch(c, KID_UPDATE, [&] { k_update<P, true><<<K, BLOCK, 0, s>>>(c->Qtp, c->FT, m, st, JQ, c->G, Qt_out, c->T); }));
+    else
+        TRY(launch(c, KID_UPDATE, [&] { k_update<P, false><<<K, BLOCK, 0, s>>>(c->Qtp, c->FT, m, st, JQ, c->G, Qt_out, c->T); }));
     return DSEM_OK;
 }
 
@@ -776,9 +794,10 @@ int run_dtcalc(dsem_ctx *c, unsigned long long *out)
     default: return set_err(ctx, DSEM_ERR_ARG, "p");    \
     }
 
-int rhs_dispatch(dsem_ctx *c, double *JQ, const StageArgs &st, double *Qt_out)
+int rhs_dispatch(dsem_ctx *c, double *JQ, const StageArgs &st, double *Qt_out, bool traces_ready = false,
+                 bool make_traces = false)
 {
-    DISPATCH_P(c, run_rhs, JQ, st, Qt_out);
+    DISPATCH_P(c, run_rhs, JQ, st, Qt_out, traces_ready, make_traces);
 }
 
 int dtcalc_dispatch(dsem_ctx *c, unsigned long long *out)
@@ -1053,7 +1072,9 @@ int dsem_rk_steps(dsem_ctx *c, double t, double dt, int nsteps, double *resid_ou
             const bool last = check && s == nsteps - 1 && k == 4;
             st.resid = last ? &c->d_scalars[0] : nullptr;
             st.nonfinite = check ? reinterpret_cast<unsigned int *>(&c->d_scalars[2]) : nullptr;
-            TRY(rhs_dispatch(c, c->JQ, st, nullptr));
+            const bool ready = !(s == 0 && k == 0);
+            const bool make = !(s == nsteps - 1 && k == 4);
+            TRY(rhs_dispatch(c, c->JQ, st, nullptr, ready, make));
         }
     }
     if (check) {
