@@ -21,6 +21,7 @@ DSEM_ERR_NOMEM = -5
 
 PREC_FP64, PREC_FP32, PREC_BF16X3 = 0, 1, 2
 METRICS_AUTO, METRICS_PER_POINT, METRICS_PER_ELEMENT = 0, 1, 2
+EV_DUCROS, EV_SMOOTHING, EV_CMAX_CAP = 1, 2, 4
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int32)
@@ -34,7 +35,7 @@ class MeshDesc(C.Structure):
 
 class Params(C.Structure):
     _fields_ = [("gamma", C.c_double), ("re", C.c_double), ("pr", C.c_double), ("mach", C.c_double),
-                ("shock", C.c_int32), ("precision", C.c_int32), ("metrics", C.c_int32), ("reserved", C.c_int32),
+                ("shock", C.c_int32), ("precision", C.c_int32), ("metrics", C.c_int32), ("ev_flags", C.c_int32),
                 ("cmu", C.c_double), ("mumax", C.c_double), ("cmax", C.c_double)]
 
 
@@ -64,6 +65,10 @@ def load() -> C.CDLL:
         "dsem_set_state": ([vp, _dp, C.c_int], C.c_int),
         "dsem_get_state": ([vp, _dp, C.c_int], C.c_int),
         "dsem_set_viscosity": ([vp, _dp], C.c_int),
+        "dsem_get_viscosity": ([vp, _dp, _dp], C.c_int),
+        "dsem_ev_update": ([vp, _dp, _dp, _dp], C.c_int),
+        "dsem_set_lagged_states": ([vp, _dp], C.c_int),
+        "dsem_get_lagged_states": ([vp, _dp], C.c_int),
         "dsem_rhs": ([vp, C.c_double, _dp], C.c_int),
         "dsem_rk_step": ([vp, C.c_double, C.c_double, _dp], C.c_int),
         "dsem_rk_steps": ([vp, C.c_double, C.c_double, C.c_int, _dp], C.c_int),

@@ -301,6 +301,7 @@ __device__ __forceinline__ void load_physical_state(const double *__restrict__ J
 template <int P, int D>
 __device__ __forceinline__ void volume_grid(const DevMesh &m, const DevPhys &ph, const double *__restrict__ QA,
                                             const double *__restrict__ muh, double *__restrict__ FV,
+                                            double *__restrict__ Slag,
                                             const double *sQ, const double *sG, double *sF, int e,
                                             const int *mort, double (&acc)[(Sz<P>::NP + VBLOCK - 1) / VBLOCK][5])
 {
@@ -336,6 +337,10 @@ __device__ __forceinline__ void volume_grid(const DevMesh &m, const DevPhys &ph,
                 Qv[v] = t;
                 if (v == 1 || v == 3) __builtin_amdgcn_sched_barrier(0);
             }
+        }
+        if (Slag) {
+#pragma unroll
+            for (int v = 0; v < 5; ++v) Slag[(((size_t)e * 3 + D) * NEQ + v) * S::NS + s] = Qv[v];
         }
         const Prim pr = primitives(ph, Qv);
         __builtin_amdgcn_sched_barrier(0);
@@ -407,7 +412,8 @@ __device__ __forceinline__ void volume_grid(const DevMesh &m, const DevPhys &ph,
 template <int P>
 __global__ __launch_bounds__(VBLOCK) void k_volume(const double *__restrict__ JQ, DevMesh m, DevPhys ph,
                                                     const double *__restrict__ QA, const double *__restrict__ muh,
-                                                    double *__restrict__ FV, double *__restrict__ Qtp)
+                                                    double *__restrict__ FV, double *__restrict__ Qtp,
+                                                    double *__restrict__ Slag)
 {
     using S = Sz<P>;
     constexpr int R = (S::NP + VBLOCK - 1) / VBLOCK;
@@ -475,9 +481,9 @@ __global__ __launch_bounds__(VBLOCK) void k_volume(const double *__restrict__ JQ
     for (int r = 0; r < R; ++r)
 #pragma unroll
         for (int v = 0; v < 5; ++v) acc[r][v] = 0.0;
-    volume_grid<P, 0>(m, ph, QA, muh, FV, sQ, sG, sF, e, mort, acc);
-    volume_grid<P, 1>(m, ph, QA, muh, FV, sQ, sG, sF, e, mort, acc);
-    volume_grid<P, 2>(m, ph, QA, muh, FV, sQ, sG, sF, e, mort, acc);
+    volume_grid<P, 0>(m, ph, QA, muh, FV, Slag, sQ, sG, sF, e, mort, acc);
+    volume_grid<P, 1>(m, ph, QA, muh, FV, Slag, sQ, sG, sF, e, mort, acc);
+    volume_grid<P, 2>(m, ph, QA, muh, FV, Slag, sQ, sG, sF, e, mort, acc);
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         const int pt = tid + r * VBLOCK;
@@ -588,6 +594,209 @@ __global__ __launch_bounds__(BLOCK) void k_update(const double *__restrict__ Qtp
     }
 }
 
+// ------------------------------------------------------------ entropy viscosity
+struct EvArgs {
+    double cmu, mumax, cmax;
+    int ducros, smoothing, cmax_cap;
+};
+
+// One smoothing update of entropy_viscosity_smooting (viscosity_smoothing.f90:4-195)
+// at Gauss point (n,m,l) (1-based), reproducing the reference's term order
+// for the corner (kind 0), face (kind 1..6: n=1, m=1, l=1, n=N, m=N, l=N)
+// and interior (kind 7) formulas.
+template <int P>
+__device__ __forceinline__ double ev_smooth_point(const double *mu, int kind, int n, int m, int l)
+{
+    using S = Sz<P>;
+    constexpr int N = S::NCG;
+    auto MU = [&](int a, int b, int c) { return mu[(a - 1) + N * ((b - 1) + N * (c - 1))]; };
+    const double SIXTH = 1.0 / 6.0, TENTH = 0.10, TWELVETH = 1.0 / 12.0;
+    if (kind == 0) {
+        const int dn = (n == 1) ? 1 : -1, dm = (m == 1) ? 1 : -1, dl = (l == 1) ? 1 : -1;
+        return SIXTH * (3.0 * MU(n, m, l) + MU(n + dn, m, l) + MU(n, m + dm, l) + MU(n, m, l + dl));
+    }
+    if (kind == 1) return TENTH * (5.0 * MU(n, m, l) + MU(n + 1, m, l) + MU(n, m - 1, l) + MU(n, m + 1, l) + MU(n, m, l + 1) + MU(n, m, l - 1));
+    if (kind == 2) return TENTH * (5.0 * MU(n, m, l) + MU(n, m + 1, l) + MU(n - 1, m, l) + MU(n + 1, m, l) + MU(n, m, l + 1) + MU(n, m, l - 1));
+    if (kind == 3) return TENTH * (5.0 * MU(n, m, l) + MU(n, m, l + 1) + MU(n, m - 1, l) + MU(n, m + 1, l) + MU(n - 1, m, l) + MU(n + 1, m, l));
+    if (kind == 4) return TENTH * (5.0 * MU(n, m, l) + MU(n - 1, m, l) + MU(n, m - 1, l) + MU(n, m + 1, l) + MU(n, m, l + 1) + MU(n, m, l - 1));
+    if (kind == 5) return TENTH * (5.0 * MU(n, m, l) + MU(n, m - 1, l) + MU(n - 1, m, l) + MU(n + 1, m, l) + MU(n, m, l + 1) + MU(n, m, l - 1));
+    if (kind == 6) return TENTH * (5.0 * MU(n, m, l) + MU(n, m, l - 1) + MU(n, m - 1, l) + MU(n, m + 1, l) + MU(n - 1, m, l) + MU(n + 1, m, l));
+    return TWELVETH * (6.0 * MU(n, m, l) + MU(n + 1, m, l) + MU(n - 1, m, l) + MU(n, m + 1, l) + MU(n, m - 1, l) + MU(n, m, l + 1) + MU(n, m, l - 1));
+}
+
+// Entropy (EV_routines/Main/Driver.f90:21-60) for one element:
+//   RHS_ent_trans (RHS.f90:21-190) from the lagged face-averaged staggered
+//   states Slag and the current state: D = (Phi + Gamma)/(T Re) and the
+//   Ducros sensor (calculate_terms, RHS.f90:207-278);
+//   Entropy_Viscosity (EViscosity_GL.f90:19-101): mu = min(max(Cmu Re rho h^2 |D|, 0), mumax)
+//   (times the Ducros sensor when enabled; with cmax_cap the cap is
+//   cmax * max(rho) * h * max(|u| + c) over the element instead of mumax);
+//   optional smoothing; EV_Prolong (Prolong_gtol.f90:7-51) to the three grids.
+// The second-derivative term and the density sensor of RHS_ent_trans, and
+// the Strain routine, feed no output and are not computed.
+template <int P>
+__global__ __launch_bounds__(BLOCK) void k_ev(const double *__restrict__ JQ, const double *__restrict__ Slag, DevMesh m,
+                                               DevPhys ph, EvArgs ev, double *__restrict__ muh,
+                                               double *__restrict__ muhg_out, double *__restrict__ resid_out,
+                                               double *__restrict__ ducros_out)
+{
+    using S = Sz<P>;
+    __shared__ double sS[3 * NEQ * S::NS];
+    __shared__ double sT[3 * S::NS];
+    __shared__ double sMu[S::NP];
+    __shared__ double sRed[2 * BLOCK];
+    const int e = blockIdx.x, tid = threadIdx.x;
+    const double gamma = ph.gamma;
+    const double *src = Slag + (size_t)e * 3 * NEQ * S::NS;
+    for (int i = tid; i < 3 * NEQ * S::NS; i += BLOCK) sS[i] = src[i];
+    __syncthreads();
+    for (int i = tid; i < 3 * S::NS; i += BLOCK) {
+        const int d = i / S::NS, s2 = i % S::NS;
+        const double *q = sS + d * NEQ * S::NS;
+        const double q1 = q[s2];
+        const double a1 = q[S::NS + s2] / q1, a2 = q[2 * S::NS + s2] / q1, a3 = q[3 * S::NS + s2] / q1;
+        const double Pv = (gamma - 1.0) * (q[4 * S::NS + s2] - 0.5 * q1 * (a1 * a1 + a2 * a2 + a3 * a3));
+        sT[i] = Pv * gamma / q1;
+    }
+    __syncthreads();
+    const int npm = m.ms ? S::NP : 1;
+    const double *gg = m.gmetg + (size_t)e * 9 * npm;
+    const double *jq = JQ + (size_t)e * NEQ * S::NP;
+    const double *jac = m.jacob + (size_t)e * npm;
+    const double c3 = pow((double)(S::NCG * S::NCG * S::NCG), 1.0 / 3.0);
+    double rhomax = 0.0, utmax = 0.0;
+    for (int pt = tid; pt < S::NP; pt += BLOCK) {
+        const int n = pt % S::NCG, mi = (pt / S::NCG) % S::NCG, l = pt / (S::NCG * S::NCG);
+        const int pq = pt * m.ms;
+        double gr[6][3];
+#pragma unroll
+        for (int f = 0; f < 6; ++f) {
+            double w[3];
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                const int k = d == 0 ? n : (d == 1 ? mi : l);
+                const int a = d == 0 ? mi : n;
+                const int b = d == 2 ? mi : l;
+                const double *fld = (f < 5) ? sS + (d * NEQ + f) * S::NS : sT + d * S::NS;
+                double t = 0.0;
+#pragma unroll
+                for (int j = 0; j < S::NC; ++j) t = fma(c_D[P][k * S::NC + j], fld[sidx<P>(d, j, a, b)], t);
+                w[d] = t;
+            }
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+                gr[f][c] = gg[(0 * 3 + c) * npm + pq] * w[0] + gg[(1 * 3 + c) * npm + pq] * w[1] + gg[(2 * 3 + c) * npm + pq] * w[2];
+        }
+        const double jc = jac[pq];
+        double Q[5];
+#pragma unroll
+        for (int v = 0; v < 5; ++v) Q[v] = jq[v * S::NP + pt] * jc;
+        const double rho = Q[0];
+        const double u = Q[1] / rho, v = Q[2] / rho, w = Q[3] / rho;
+        const double p = (gamma - 1.0) * (Q[4] - 0.5 * rho * (u * u + v * v + w * w));
+        const double T = p * gamma / rho;
+        const double ux = (gr[1][0] - u * gr[0][0]) / rho, uy = (gr[1][1] - u * gr[0][1]) / rho, uz = (gr[1][2] - u * gr[0][2]) / rho;
+        const double vx = (gr[2][0] - v * gr[0][0]) / rho, vy = (gr[2][1] - v * gr[0][1]) / rho, vz = (gr[2][2] - v * gr[0][2]) / rho;
+        const double wx = (gr[3][0] - w * gr[0][0]) / rho, wy = (gr[3][1] - w * gr[0][1]) / rho, wz = (gr[3][2] - w * gr[0][2]) / rho;
+        const double tx = gr[5][0], ty = gr[5][1], tz = gr[5][2];
+        const double s1 = ux + vy + wz;
+        const double phi = (2.0 * (ux * ux + vy * vy + wz * wz) + ((vx + uy) * (vx + uy) + (uz + wx) * (uz + wx) + (vz + wy) * (vz + wy))
+                            - 2.0 * (s1 * s1) / 3.0);
+        const double gm = (tx * tx + ty * ty + tz * tz) / (T * (gamma - 1.0) * ph.pr);
+        const double cur = sqrt((wy - vz) * (wy - vz) + (uz - wx) * (uz - wx) + (vx - uy) * (vx - uy));
+        const double rhs = (((phi + gm) / T) / ph.re);
+        const double duc = fmax(0.0, fmin(1.0, (s1 * s1 / (s1 * s1 + cur * cur + 1.0e-9))));
+        if (resid_out) resid_out[(size_t)e * S::NP + pt] = rhs;
+        if (ducros_out) ducros_out[(size_t)e * S::NP + pt] = duc;
+        const double h = pow(fabs(1.0 / jc), 1.0 / 3.0) / c3;
+        double mur = ev.cmu * ph.re * rho * (h * h) * fabs(rhs);
+        if (ev.ducros) mur = mur * duc;
+        sMu[pt] = mur;
+        if (ev.cmax_cap) {
+            rhomax = fmax(rhomax, rho);
+            utmax = fmax(utmax, sqrt(u * u + v * v + w * w) + sqrt(gamma * p / rho));
+        }
+    }
+    double cap = ev.mumax;
+    if (ev.cmax_cap) {
+        sRed[tid] = rhomax;
+        sRed[BLOCK + tid] = utmax;
+        __syncthreads();
+        for (int st = BLOCK / 2; st > 0; st >>= 1) {
+            if (tid < st) {
+                sRed[tid] = fmax(sRed[tid], sRed[tid + st]);
+                sRed[BLOCK + tid] = fmax(sRed[BLOCK + tid], sRed[BLOCK + tid + st]);
+            }
+            __syncthreads();
+        }
+    }
+    __syncthreads();
+    for (int pt = tid; pt < S::NP; pt += BLOCK) {
+        if (ev.cmax_cap) {
+            const double h = pow(fabs(1.0 / jac[pt * m.ms]), 1.0 / 3.0) / c3;
+            cap = ev.cmax * sRed[0] * h * sRed[BLOCK];
+        }
+        sMu[pt] = fmin(fmax(sMu[pt], 0.0), cap);
+    }
+    __syncthreads();
+    if (ev.smoothing) {
+        constexpr int N = S::NCG;
+        // corners (independent of each other): all eight read, then write
+        double cval = 0.0;
+        int cidx = -1;
+        if (tid < 8) {
+            const int n = (tid & 1) ? N : 1, mm = (tid & 2) ? N : 1, l = (tid & 4) ? N : 1;
+            cval = ev_smooth_point<P>(sMu, 0, n, mm, l);
+            cidx = (n - 1) + N * ((mm - 1) + N * (l - 1));
+        }
+        __syncthreads();
+        if (cidx >= 0) sMu[cidx] = cval;
+        __syncthreads();
+        // faces: the six faces are independent; within a face, lexicographic
+        // Gauss-Seidel equals a sweep over the diagonals u+v = h
+        if (N > 2) {
+            for (int hh = 0; hh <= 2 * (N - 3); ++hh) {
+                for (int i = tid; i < 6 * (N - 2); i += BLOCK) {
+                    const int f = i / (N - 2), uu = 2 + i % (N - 2), vv = 2 + hh - (uu - 2);
+                    if (vv >= 2 && vv <= N - 1) {
+                        int n, mm, l;
+                        if (f == 0) { n = 1; mm = uu; l = vv; }
+                        else if (f == 1) { n = uu; mm = 1; l = vv; }
+                        else if (f == 2) { n = uu; mm = vv; l = 1; }
+                        else if (f == 3) { n = N; mm = uu; l = vv; }
+                        else if (f == 4) { n = uu; mm = N; l = vv; }
+                        else { n = uu; mm = vv; l = N; }
+                        sMu[(n - 1) + N * ((mm - 1) + N * (l - 1))] = ev_smooth_point<P>(sMu, 1 + f, n, mm, l);
+                    }
+                }
+                __syncthreads();
+            }
+            // interior: sweep over the planes n+m+l = const
+            for (int hh = 0; hh <= 3 * (N - 3); ++hh) {
+                for (int i = tid; i < (N - 2) * (N - 2); i += BLOCK) {
+                    const int n = 2 + i % (N - 2), mm = 2 + i / (N - 2), l = 2 + hh - (n - 2) - (mm - 2);
+                    if (l >= 2 && l <= N - 1)
+                        sMu[(n - 1) + N * ((mm - 1) + N * (l - 1))] = ev_smooth_point<P>(sMu, 7, n, mm, l);
+                }
+                __syncthreads();
+            }
+        }
+    }
+    if (muhg_out)
+        for (int pt = tid; pt < S::NP; pt += BLOCK) muhg_out[(size_t)e * S::NP + pt] = sMu[pt];
+    for (int i = tid; i < 3 * S::NS; i += BLOCK) {
+        const int d = i / S::NS, s2 = i % S::NS;
+        int j, a, b;
+        if (d == 0) { j = s2 % S::NC; a = (s2 / S::NC) % S::NCG; b = s2 / (S::NC * S::NCG); }
+        else if (d == 1) { a = s2 % S::NCG; j = (s2 / S::NCG) % S::NC; b = s2 / (S::NCG * S::NC); }
+        else { a = s2 % S::NCG; b = (s2 / S::NCG) % S::NCG; j = s2 / (S::NCG * S::NCG); }
+        double t = 0.0;
+#pragma unroll
+        for (int k = 0; k < S::NCG; ++k) t = fma(c_B[P][j * S::NCG + k], sMu[gidx<P>(d, k, a, b)], t);
+        muh[(size_t)e * 3 * S::NS + i] = t;
+    }
+}
+
 // dtcalc (dtcalc.f90:9-58): per point 1/dti, reduced to the minimum
 template <int P>
 __global__ __launch_bounds__(BLOCK) void k_dtcalc(const double *__restrict__ JQ, DevMesh m, DevPhys ph,
@@ -626,9 +835,9 @@ __global__ __launch_bounds__(BLOCK) void k_dtcalc(const double *__restrict__ JQ,
 // ====================================================================== host
 namespace {
 
-enum KernelId { KID_TRACES = 0, KID_FACE_INV, KID_VOLUME, KID_FACE_VISC, KID_UPDATE, KID_DTCALC, KID_COUNT };
+enum KernelId { KID_TRACES = 0, KID_FACE_INV, KID_VOLUME, KID_FACE_VISC, KID_UPDATE, KID_DTCALC, KID_EV, KID_COUNT };
 const char *const kKernelNames[KID_COUNT] = {"k_traces", "k_face_inv", "k_volume", "k_face_visc",
-                                             "k_update", "k_dtcalc"};
+                                             "k_update", "k_dtcalc", "k_ev"};
 
 template <typename T>
 struct DevBuf {
@@ -652,6 +861,8 @@ struct dsem_ctx {
     double *T = nullptr, *FV = nullptr, *QA = nullptr, *FI = nullptr, *FT = nullptr;
     double *jacob = nullptr, *gmetg = nullptr, *gmet[3] = {nullptr, nullptr, nullptr};
     double *nhat = nullptr, *norm = nullptr, *muh = nullptr;
+    double *Slag = nullptr, *muhg = nullptr, *ev_resid = nullptr, *ev_ducros = nullptr;   // entropy viscosity
+    bool have_lag = false;
     int *e2m = nullptr, *m_elem = nullptr, *m_face = nullptr, *m_sign = nullptr, *m_nsign = nullptr;
     unsigned long long *d_scalars = nullptr;   // [0] resid bits, [1] dt bits, [2] nonfinite flag
     std::vector<void *> allocations;
@@ -753,7 +964,8 @@ int launch(dsem_ctx *c, int kid, F &&fn)
 // previous stage's update); make_traces: this stage's update also computes
 // the traces of the updated state for the next stage.
 template <int P>
-int run_rhs(dsem_ctx *c, double *JQ, const StageArgs &st, double *Qt_out, bool traces_ready, bool make_traces)
+int run_rhs(dsem_ctx *c, double *JQ, const StageArgs &st, double *Qt_out, bool traces_ready, bool make_traces,
+            bool save_lag)
 {
     using S = Sz<P>;
     const DevMesh m = dev_mesh(c);
@@ -764,7 +976,8 @@ int run_rhs(dsem_ctx *c, double *JQ, const StageArgs &st, double *Qt_out, bool t
     hipStream_t s = c->stream;
     if (!traces_ready) TRY(launch(c, KID_TRACES, [&] { k_traces<P><<<K, BLOCK, 0, s>>>(JQ, m, c->T); }));
     TRY(launch(c, KID_FACE_INV, [&] { k_face_inv<P><<<face_blocks, BLOCK, 0, s>>>(c->T, m, ph, c->FI, c->QA); }));
-    TRY(launch(c, KID_VOLUME, [&] { k_volume<P><<<K, VBLOCK, 0, s>>>(JQ, m, ph, c->QA, muh, c->FV, c->Qtp); }));
+    double *slag = save_lag ? c->Slag : nullptr;
+    TRY(launch(c, KID_VOLUME, [&] { k_volume<P><<<K, VBLOCK, 0, s>>>(JQ, m, ph, c->QA, muh, c->FV, c->Qtp, slag); }));
     TRY(launch(c, KID_FACE_VISC, [&] { k_face_visc<P><<<face_blocks, BLOCK, 0, s>>>(c->FV, c->FI, m, ph, c->FT); }));
     if (make_traces)
         TRY(launch(c, KID_UPDATE, [&] { k_update<P, true><<<K, BLOCK, 0, s>>>(c->Qtp, c->FT, m, st, JQ, c->G, Qt_out, c->T); }));
@@ -795,9 +1008,45 @@ int run_dtcalc(dsem_ctx *c, unsigned long long *out)
     }
 
 int rhs_dispatch(dsem_ctx *c, double *JQ, const StageArgs &st, double *Qt_out, bool traces_ready = false,
-                 bool make_traces = false)
+                 bool make_traces = false, bool save_lag = false)
 {
-    DISPATCH_P(c, run_rhs, JQ, st, Qt_out, traces_ready, make_traces);
+    DISPATCH_P(c, run_rhs, JQ, st, Qt_out, traces_ready, make_traces, save_lag);
+}
+
+template <int P>
+int run_ev(dsem_ctx *c, bool debug)
+{
+    const DevMesh m = dev_mesh(c);
+    const DevPhys ph = dev_phys(c);
+    EvArgs ev;
+    ev.cmu = c->params.cmu;
+    ev.mumax = c->params.mumax;
+    ev.cmax = c->params.cmax;
+    ev.ducros = (c->params.ev_flags & DSEM_EV_DUCROS) != 0;
+    ev.smoothing = (c->params.ev_flags & DSEM_EV_SMOOTHING) != 0;
+    ev.cmax_cap = (c->params.ev_flags & DSEM_EV_CMAX_CAP) != 0;
+    return launch(c, KID_EV, [&] {
+        k_ev<P><<<c->K, BLOCK, 0, c->stream>>>(c->JQ, c->Slag, m, ph, ev, c->muh, c->muhg, debug ? c->ev_resid : nullptr,
+                                               debug ? c->ev_ducros : nullptr);
+    });
+}
+
+int ev_dispatch(dsem_ctx *c, bool debug)
+{
+    DISPATCH_P(c, run_ev, debug);
+}
+
+// Entropy at the start of a step: mu_h from the lagged states, or zero on a
+// fresh start (the reference divides 0/0 there and its max(NaN,0) yields 0).
+int ev_step(dsem_ctx *c, bool debug)
+{
+    if (!c->params.shock) return DSEM_OK;
+    if (!c->have_lag) {
+        HIP_TRY(c, hipMemsetAsync(c->muh, 0, sizeof(double) * (size_t)c->K * 3 * c->ns, c->stream));
+        HIP_TRY(c, hipMemsetAsync(c->muhg, 0, sizeof(double) * (size_t)c->K * c->np, c->stream));
+        return DSEM_OK;
+    }
+    return ev_dispatch(c, debug);
 }
 
 int dtcalc_dispatch(dsem_ctx *c, unsigned long long *out)
@@ -912,10 +1161,17 @@ int create_impl(const dsem_mesh_desc *mesh, const dsem_params *prm, int device, 
     TRY(dev_alloc(c, &c->FI, (size_t)c->M * NEQ * c->nf));
     TRY(dev_alloc(c, &c->FT, (size_t)c->M * NEQ * c->nf));
     TRY(dev_alloc(c, &c->muh, (size_t)c->K * 3 * c->ns));
+    TRY(dev_alloc(c, &c->muhg, (size_t)c->K * c->np));
+    if (c->params.shock) {
+        TRY(dev_alloc(c, &c->Slag, (size_t)c->K * 3 * NEQ * c->ns));
+        TRY(dev_alloc(c, &c->ev_resid, (size_t)c->K * c->np));
+        TRY(dev_alloc(c, &c->ev_ducros, (size_t)c->K * c->np));
+    }
     TRY(dev_alloc(c, &c->d_scalars, 4));
     HIP_TRY(c, hipMemset(c->JQ, 0, sizeof(double) * nst));
     HIP_TRY(c, hipMemset(c->G, 0, sizeof(double) * nst));
     HIP_TRY(c, hipMemset(c->muh, 0, sizeof(double) * (size_t)c->K * 3 * c->ns));
+    HIP_TRY(c, hipMemset(c->muhg, 0, sizeof(double) * (size_t)c->K * c->np));
     return DSEM_OK;
 }
 
@@ -1000,6 +1256,7 @@ int dsem_set_state(dsem_ctx *c, const double *JQ, int layout)
     HIP_TRY(c, hipSetDevice(c->device));
     HIP_TRY(c, hipMemcpyAsync(c->JQ, src, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
+    c->have_lag = false;   // a new state starts without lagged staggered states
     return DSEM_OK;
 }
 
@@ -1026,6 +1283,57 @@ int dsem_set_viscosity(dsem_ctx *c, const double *muh)
     if (!c || !c->stream || !muh) return set_err(c, DSEM_ERR_ARG, "set_viscosity arguments");
     HIP_TRY(c, hipSetDevice(c->device));
     HIP_TRY(c, hipMemcpyAsync(c->muh, muh, sizeof(double) * (size_t)c->K * 3 * c->ns, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return DSEM_OK;
+}
+
+int dsem_get_viscosity(dsem_ctx *c, double *muhg, double *muh)
+{
+    if (!c || !c->stream) return set_err(c, DSEM_ERR_ARG, "get_viscosity arguments");
+    HIP_TRY(c, hipSetDevice(c->device));
+    if (muhg)
+        HIP_TRY(c, hipMemcpyAsync(muhg, c->muhg, sizeof(double) * (size_t)c->K * c->np, hipMemcpyDeviceToHost, c->stream));
+    if (muh)
+        HIP_TRY(c, hipMemcpyAsync(muh, c->muh, sizeof(double) * (size_t)c->K * 3 * c->ns, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return DSEM_OK;
+}
+
+int dsem_ev_update(dsem_ctx *c, double *muhg, double *resid, double *ducros)
+{
+    if (!c || !c->stream) return set_err(c, DSEM_ERR_ARG, "ev_update arguments");
+    if (!c->params.shock) return set_err(c, DSEM_ERR_ARG, "entropy viscosity needs params.shock = 1");
+    HIP_TRY(c, hipSetDevice(c->device));
+    TRY(ev_step(c, true));
+    const size_t n = (size_t)c->K * c->np;
+    if (muhg) HIP_TRY(c, hipMemcpyAsync(muhg, c->muhg, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+    if (resid && c->have_lag) HIP_TRY(c, hipMemcpyAsync(resid, c->ev_resid, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+    if (ducros && c->have_lag) HIP_TRY(c, hipMemcpyAsync(ducros, c->ev_ducros, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if (!c->have_lag) {
+        if (resid) std::memset(resid, 0, sizeof(double) * n);
+        if (ducros) std::memset(ducros, 0, sizeof(double) * n);
+    }
+    return DSEM_OK;
+}
+
+int dsem_set_lagged_states(dsem_ctx *c, const double *stag)
+{
+    if (!c || !c->stream || !stag || !c->Slag) return set_err(c, DSEM_ERR_ARG, "set_lagged_states needs shock = 1");
+    HIP_TRY(c, hipSetDevice(c->device));
+    HIP_TRY(c, hipMemcpyAsync(c->Slag, stag, sizeof(double) * (size_t)c->K * 3 * NEQ * c->ns, hipMemcpyHostToDevice,
+                              c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    c->have_lag = true;
+    return DSEM_OK;
+}
+
+int dsem_get_lagged_states(dsem_ctx *c, double *stag)
+{
+    if (!c || !c->stream || !stag || !c->Slag) return set_err(c, DSEM_ERR_ARG, "get_lagged_states needs shock = 1");
+    HIP_TRY(c, hipSetDevice(c->device));
+    HIP_TRY(c, hipMemcpyAsync(stag, c->Slag, sizeof(double) * (size_t)c->K * 3 * NEQ * c->ns, hipMemcpyDeviceToHost,
+                              c->stream));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     return DSEM_OK;
 }
@@ -1063,6 +1371,7 @@ int dsem_rk_steps(dsem_ctx *c, double t, double dt, int nsteps, double *resid_ou
     const bool check = resid_out != nullptr;
     if (check) HIP_TRY(c, hipMemsetAsync(c->d_scalars, 0, sizeof(unsigned long long) * 4, c->stream));
     for (int s = 0; s < nsteps; ++s) {
+        TRY(ev_step(c, false));
         for (int k = 0; k < 5; ++k) {
             StageArgs st;
             st.a = ark[k];
@@ -1074,8 +1383,9 @@ int dsem_rk_steps(dsem_ctx *c, double t, double dt, int nsteps, double *resid_ou
             st.nonfinite = check ? reinterpret_cast<unsigned int *>(&c->d_scalars[2]) : nullptr;
             const bool ready = !(s == 0 && k == 0);
             const bool make = !(s == nsteps - 1 && k == 4);
-            TRY(rhs_dispatch(c, c->JQ, st, nullptr, ready, make));
+            TRY(rhs_dispatch(c, c->JQ, st, nullptr, ready, make, c->params.shock && k == 4));
         }
+        if (c->params.shock) c->have_lag = true;
     }
     if (check) {
         unsigned long long h[4];

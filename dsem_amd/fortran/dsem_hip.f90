@@ -20,6 +20,7 @@ module dsem_hip
    integer(c_int), parameter :: DSEM_ERR_NOMEM = -5
    integer(c_int), parameter :: DSEM_PREC_FP64 = 0, DSEM_PREC_FP32 = 1, DSEM_PREC_BF16X3 = 2
    integer(c_int), parameter :: DSEM_METRICS_AUTO = 0, DSEM_METRICS_PER_POINT = 1, DSEM_METRICS_PER_ELEMENT = 2
+   integer(c_int), parameter :: DSEM_EV_DUCROS = 1, DSEM_EV_SMOOTHING = 2, DSEM_EV_CMAX_CAP = 4
 
    type, bind(C) :: dsem_mesh_desc
       integer(c_int32_t) :: K, p, nmortar, reserved
@@ -29,7 +30,7 @@ module dsem_hip
 
    type, bind(C) :: dsem_params
       real(c_double) :: gamma, re, pr, mach
-      integer(c_int32_t) :: shock, precision, metrics, reserved
+      integer(c_int32_t) :: shock, precision, metrics, ev_flags
       real(c_double) :: cmu, mumax, cmax
    end type dsem_params
 

@@ -39,7 +39,7 @@ class DSEM:
     def __init__(self, mesh: Mesh, gamma: float = 1.4, re: float = 1600.0, pr: float = 0.71,
                  mach: float = 1.0, shock: bool = False, precision: str = "fp64",
                  metrics: str = "auto", device: int = 0, cmu: float = 1.0, mumax: float = 1.0,
-                 cmax: float = 0.5):
+                 cmax: float = 0.5, ducros: bool = False, smoothing: bool = False, cmax_cap: bool = False):
         self._L = L.load()
         self.mesh = mesh
         self._keep = [np.ascontiguousarray(mesh.mortar_elem, dtype=np.int32),
@@ -49,7 +49,9 @@ class DSEM:
                       np.ascontiguousarray(mesh.corners, dtype=np.float64)]
         me, mf, ms, mn, cr = self._keep
         desc = L.MeshDesc(mesh.K, mesh.p, mesh.nmortar, 0, _ip(me), _ip(mf), _ip(ms), _ip(mn), None, _dp(cr))
-        prm = L.Params(gamma, re, pr, mach, int(bool(shock)), _PREC[precision], _METRICS[metrics], 0,
+        flags = (L.EV_DUCROS if ducros else 0) | (L.EV_SMOOTHING if smoothing else 0) | \
+            (L.EV_CMAX_CAP if cmax_cap else 0)
+        prm = L.Params(gamma, re, pr, mach, int(bool(shock)), _PREC[precision], _METRICS[metrics], flags,
                        cmu, mumax, cmax)
         h = C.c_void_p()
         rc = self._L.dsem_create(C.byref(desc), C.byref(prm), device, C.byref(h))
@@ -116,6 +118,31 @@ class DSEM:
         if muh.size != self.K * 3 * self.ns:
             raise ValueError("muh must be [K][3][nc*ncg^2]")
         self._check(self._L.dsem_set_viscosity(self._h, _dp(muh)))
+
+    def get_viscosity(self):
+        """(mu_h at the Gauss points [K, np], mu_h on the staggered grids [K, 3, ns])."""
+        g = np.zeros((self.K, self.np))
+        st = np.zeros((self.K, 3, self.ns))
+        self._check(self._L.dsem_get_viscosity(self._h, _dp(g), _dp(st)))
+        return g, st
+
+    def ev_update(self):
+        """Entropy (EV_routines/Main/Driver.f90:21) on the current state; returns
+        (mu_h, residual D, Ducros sensor) at the Gauss points."""
+        mu, r, dc = (np.zeros((self.K, self.np)) for _ in range(3))
+        self._check(self._L.dsem_ev_update(self._h, _dp(mu), _dp(r), _dp(dc)))
+        return mu, r, dc
+
+    def set_lagged_states(self, stag: np.ndarray):
+        stag = np.ascontiguousarray(stag, dtype=np.float64)
+        if stag.size != self.K * 3 * 5 * self.ns:
+            raise ValueError("lagged states must be [K][3][5][nc*ncg^2]")
+        self._check(self._L.dsem_set_lagged_states(self._h, _dp(stag)))
+
+    def get_lagged_states(self) -> np.ndarray:
+        out = np.zeros((self.K, 3, 5, self.ns))
+        self._check(self._L.dsem_get_lagged_states(self._h, _dp(out)))
+        return out
 
     # ------------------------------------------------------------------ path
     def rhs(self, t: float = 0.0) -> np.ndarray:

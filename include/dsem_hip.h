@@ -82,13 +82,19 @@ typedef struct {
     const double  *corners;        /* [K][8][3] hex8 corners, FE order        */
 } dsem_mesh_desc;
 
+#define DSEM_EV_DUCROS    1   /* multiply mu_h by the Ducros sensor ('ducrosSens') */
+#define DSEM_EV_SMOOTHING 2   /* smooth mu_h in each element ('Smoothing')         */
+#define DSEM_EV_CMAX_CAP  4   /* cap mu_h by cmax*max(rho)*h*max(|u|+c) per element
+                                 instead of mumax (the reference's commented-out
+                                 variant, EViscosity_GL.f90:82)                  */
+
 typedef struct {
     double  gamma, re, pr, mach;   /* physics module (mach = 1 with T0=100 gives Ma 0.1) */
-    int32_t shock;                 /* entropy viscosity added to mu (0/1)     */
+    int32_t shock;                 /* entropy viscosity, once per step (input keyword 'shock') */
     int32_t precision;             /* DSEM_PREC_*                             */
     int32_t metrics;               /* DSEM_METRICS_*                          */
-    int32_t reserved;
-    double  cmu, mumax, cmax;      /* entropy-viscosity coefficients          */
+    int32_t ev_flags;              /* DSEM_EV_*                               */
+    double  cmu, mumax, cmax;      /* entropy-viscosity coefficients (physics keywords Cmu, mumax) */
 } dsem_params;
 
 int  dsem_create(const dsem_mesh_desc *mesh, const dsem_params *params, int device, dsem_ctx **out);
@@ -106,8 +112,20 @@ int  dsem_set_state(dsem_ctx *ctx, const double *JQ, int layout);
 int  dsem_get_state(dsem_ctx *ctx, double *JQ, int layout);
 
 /* Entropy viscosity mu_h on the three staggered grids, [K][3][nc*ncg^2]
- * (muhlgg/muhglg/muhggl of Vis_Fluxes_new.f90:126-190), used when shock=1. */
+ * (muhlgg/muhglg/muhggl of Vis_Fluxes_new.f90:126-190), used when shock=1.
+ * dsem_rk_step recomputes it at the start of every step, as tstep calls
+ * Entropy (src/tstep.f90:36-38); dsem_rhs uses the current field. */
 int  dsem_set_viscosity(dsem_ctx *ctx, const double *muh);
+int  dsem_get_viscosity(dsem_ctx *ctx, double *muhg, double *muh);
+
+/* Entropy (src/EV_routines/Main/Driver.f90:21): mu_h from the current state
+ * and the lagged staggered states (Qlgg/Qglg/Qggl after face averaging,
+ * [K][3][5][nc*ncg^2]) that the last RK stage left behind.  Optional host
+ * outputs at the Gauss points [K][ncg^3]: mu_h, the residual D and the
+ * Ducros sensor.  Without lagged states (fresh start) mu_h is zero. */
+int  dsem_ev_update(dsem_ctx *ctx, double *muhg, double *resid, double *ducros);
+int  dsem_set_lagged_states(dsem_ctx *ctx, const double *stag);
+int  dsem_get_lagged_states(dsem_ctx *ctx, double *stag);
 
 /* Qt = d(JQ)/dt for the current state (Fluxes + update's divergence);
  * Qt_out [K][5][ncg^3] compact, or NULL to keep it on the device. */

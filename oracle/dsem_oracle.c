@@ -796,3 +796,230 @@ double or_dtcalc(const or_mesh *mesh, const or_phys *ph, const double *JQ)
     }
     return dt;
 }
+
+/* ------------------------------------------------------------------ */
+/* Entropy viscosity                                                   */
+/* ------------------------------------------------------------------ */
+/* Gauss-point gradient of one field given on the three staggered grids:
+ * Gauss_Deriv (SolnDerivsc.f90:14-83). w[d][np] scratch, out[3][np]. */
+static void gauss_deriv_field(const ops_t *o, const double *gmg, const double *s0, const double *s1,
+                              const double *s2, double *out_x, double *out_y, double *out_z)
+{
+    const int ncg = o->ncg, nc = o->nc, np = o->np;
+    double W[3][729];
+    const double *Sd[3] = {s0, s1, s2};
+    for (int d = 0; d < 3; ++d)
+        for (int b = 0; b < ncg; ++b)
+            for (int a = 0; a < ncg; ++a)
+                for (int k = 0; k < ncg; ++k) {
+                    double s = 0.0;
+                    for (int j = 0; j < nc; ++j) {
+                        double t = o->dmx[k * nc + j] * Sd[d][sidx(d, j, a, b, ncg, nc)];
+                        s = (j == 0) ? t : s + t;
+                    }
+                    W[d][gidx(d, k, a, b, ncg)] = s;
+                }
+    for (int pt = 0; pt < np; ++pt) {
+        const double *G9 = gmg + (size_t)pt * 9;
+        const double w1 = W[0][pt], w2 = W[1][pt], w3 = W[2][pt];
+        out_x[pt] = G9[0] * w1 + G9[3] * w2 + G9[6] * w3;
+        out_y[pt] = G9[1] * w1 + G9[4] * w2 + G9[7] * w3;
+        out_z[pt] = G9[2] * w1 + G9[5] * w2 + G9[8] * w3;
+    }
+}
+
+/* calculate_terms (EV_routines/Main/RHS.f90:207-278): residual and Ducros
+ * sensor.  The second-derivative term (lam) and the density sensor are
+ * computed there but reach no output of the EV path, so they are omitted. */
+static void ev_terms(const or_phys *ph, const double *Q, const double *Qx, const double *Qy, const double *Qz,
+                     const double *T1, double *rhs, double *ducros)
+{
+    const double gamma = ph->gamma;
+    const double rho = Q[0];
+    const double u = Q[1] / rho, v = Q[2] / rho, w = Q[3] / rho;
+    const double p = (gamma - 1.0) * (Q[4] - 0.5 * rho * (u * u + v * v + w * w));
+    const double T = p * gamma / rho;
+    const double ux = (Qx[1] - u * Qx[0]) / rho, uy = (Qy[1] - u * Qy[0]) / rho, uz = (Qz[1] - u * Qz[0]) / rho;
+    const double vx = (Qx[2] - v * Qx[0]) / rho, vy = (Qy[2] - v * Qy[0]) / rho, vz = (Qz[2] - v * Qz[0]) / rho;
+    const double wx = (Qx[3] - w * Qx[0]) / rho, wy = (Qy[3] - w * Qy[0]) / rho, wz = (Qz[3] - w * Qz[0]) / rho;
+    const double tx = T1[0], ty = T1[1], tz = T1[2];
+    const double s1 = ux + vy + wz;
+    const double phi = (2.0 * (ux * ux + vy * vy + wz * wz) + ((vx + uy) * (vx + uy) + (uz + wx) * (uz + wx)
+                        + (vz + wy) * (vz + wy)) - 2.0 * (s1 * s1) / 3.0);
+    const double gam = (tx * tx + ty * ty + tz * tz) / (T * (gamma - 1.0) * ph->pr);
+    const double div = ux + vy + wz;
+    const double cur = sqrt((wy - vz) * (wy - vz) + (uz - wx) * (uz - wx) + (vx - uy) * (vx - uy));
+    *rhs = (((phi + gam) / T) / ph->re);
+    *ducros = fmax(0.0, fmin(1.0, (div * div / (div * div + cur * cur + 1.0e-9))));
+}
+
+/* entropy_viscosity_smooting (EV_routines/Filter/viscosity_smoothing.f90:4-195):
+ * in-place (Gauss-Seidel order) smoothing of one element's mu at the Gauss
+ * points: corners, the six faces, then the interior. */
+static void ev_smooth(double *mu, int ncg)
+{
+#define MU(n, m, l) mu[((n) - 1) + ncg * (((m) - 1) + ncg * ((l) - 1))]
+    const double SIXTH = 1.0 / 6.0, TENTH = 0.10, TWELVETH = 1.0 / 12.0;
+    const int N = ncg;
+    int n, m, l;
+    n = 1; m = 1; l = 1;
+    MU(1, 1, 1) = SIXTH * (3.0 * MU(n, m, l) + MU(n + 1, m, l) + MU(n, m + 1, l) + MU(n, m, l + 1));
+    n = N; m = 1; l = 1;
+    MU(n, m, l) = SIXTH * (3.0 * MU(n, m, l) + MU(n - 1, m, l) + MU(n, m + 1, l) + MU(n, m, l + 1));
+    n = 1; m = N; l = 1;
+    MU(n, m, l) = SIXTH * (3.0 * MU(n, m, l) + MU(n + 1, m, l) + MU(n, m - 1, l) + MU(n, m, l + 1));
+    n = 1; m = 1; l = N;
+    MU(n, m, l) = SIXTH * (3.0 * MU(n, m, l) + MU(n + 1, m, l) + MU(n, m + 1, l) + MU(n, m, l - 1));
+    n = 1; m = N; l = N;
+    MU(n, m, l) = SIXTH * (3.0 * MU(n, m, l) + MU(n + 1, m, l) + MU(n, m - 1, l) + MU(n, m, l - 1));
+    n = N; m = 1; l = N;
+    MU(n, m, l) = SIXTH * (3.0 * MU(n, m, l) + MU(n - 1, m, l) + MU(n, m + 1, l) + MU(n, m, l - 1));
+    n = N; m = N; l = 1;
+    MU(n, m, l) = SIXTH * (3.0 * MU(n, m, l) + MU(n - 1, m, l) + MU(n, m - 1, l) + MU(n, m, l + 1));
+    n = N; m = N; l = N;
+    MU(n, m, l) = SIXTH * (3.0 * MU(n, m, l) + MU(n - 1, m, l) + MU(n, m - 1, l) + MU(n, m, l - 1));
+    n = 1;
+    for (m = 2; m <= N - 1; ++m)
+        for (l = 2; l <= N - 1; ++l)
+            MU(n, m, l) = TENTH * (5.0 * MU(n, m, l) + MU(n + 1, m, l) + MU(n, m - 1, l) + MU(n, m + 1, l)
+                                   + MU(n, m, l + 1) + MU(n, m, l - 1));
+    m = 1;
+    for (n = 2; n <= N - 1; ++n)
+        for (l = 2; l <= N - 1; ++l)
+            MU(n, m, l) = TENTH * (5.0 * MU(n, m, l) + MU(n, m + 1, l) + MU(n - 1, m, l) + MU(n + 1, m, l)
+                                   + MU(n, m, l + 1) + MU(n, m, l - 1));
+    l = 1;
+    for (n = 2; n <= N - 1; ++n)
+        for (m = 2; m <= N - 1; ++m)
+            MU(n, m, l) = TENTH * (5.0 * MU(n, m, l) + MU(n, m, l + 1) + MU(n, m - 1, l) + MU(n, m + 1, l)
+                                   + MU(n - 1, m, l) + MU(n + 1, m, l));
+    n = N;
+    for (m = 2; m <= N - 1; ++m)
+        for (l = 2; l <= N - 1; ++l)
+            MU(n, m, l) = TENTH * (5.0 * MU(n, m, l) + MU(n - 1, m, l) + MU(n, m - 1, l) + MU(n, m + 1, l)
+                                   + MU(n, m, l + 1) + MU(n, m, l - 1));
+    m = N;
+    for (n = 2; n <= N - 1; ++n)
+        for (l = 2; l <= N - 1; ++l)
+            MU(n, m, l) = TENTH * (5.0 * MU(n, m, l) + MU(n, m - 1, l) + MU(n - 1, m, l) + MU(n + 1, m, l)
+                                   + MU(n, m, l + 1) + MU(n, m, l - 1));
+    l = N;
+    for (n = 2; n <= N - 1; ++n)
+        for (m = 2; m <= N - 1; ++m)
+            MU(n, m, l) = TENTH * (5.0 * MU(n, m, l) + MU(n, m, l - 1) + MU(n, m - 1, l) + MU(n, m + 1, l)
+                                   + MU(n - 1, m, l) + MU(n + 1, m, l));
+    for (n = 2; n <= N - 1; ++n)
+        for (m = 2; m <= N - 1; ++m)
+            for (l = 2; l <= N - 1; ++l)
+                MU(n, m, l) = TWELVETH * (6.0 * MU(n, m, l) + MU(n + 1, m, l) + MU(n - 1, m, l) + MU(n, m + 1, l)
+                                          + MU(n, m - 1, l) + MU(n, m, l + 1) + MU(n, m, l - 1));
+#undef MU
+}
+
+/* RHS_ent_trans (RHS.f90:21-190) + Entropy_Viscosity (EViscosity_GL.f90:19-101)
+ * + smoothing + EV_Prolong (Prolong_gtol.f90:7-51).  delta_g: staticSmag.f90:191-201.
+ * The Strain routine called in between (EViscosity_GL.f90:314-348) stores a
+ * field that nothing on this path reads, so it is not restated. */
+int or_ev_viscosity(const or_mesh *mesh, const or_phys *ph, const or_ev *ev, const double *JQ,
+                    const double *stag_lag, double *muhg, double *muh, double *resid, double *ducros)
+{
+    ops_t o;
+    if (make_ops(mesh->p, &o)) return -1;
+    const int K = mesh->K, ncg = o.ncg, nc = o.nc, np = o.np, ns = o.ns;
+    const double gamma = ph->gamma;
+#pragma omp parallel for schedule(static)
+    for (int e = 0; e < K; ++e) {
+        double Tst[3][810], G[6][3][729], mu[729];
+        const double *S = stag_lag + (size_t)e * 3 * NEQ * ns;
+        const double *gmg = mesh->gmetg + (size_t)e * np * 9;
+        /* temperature on the staggered grids: P*gamma/rho */
+        for (int d = 0; d < 3; ++d)
+            for (int s = 0; s < ns; ++s) {
+                const double *q = S + (size_t)d * NEQ * ns;
+                const double q1 = q[0 * ns + s];
+                const double P = (gamma - 1.0) * (q[4 * ns + s] - 0.5 * q1 * ((q[1 * ns + s] / q1) * (q[1 * ns + s] / q1)
+                                  + (q[2 * ns + s] / q1) * (q[2 * ns + s] / q1) + (q[3 * ns + s] / q1) * (q[3 * ns + s] / q1)));
+                Tst[d][s] = P * gamma / q1;
+            }
+        for (int v = 0; v < NEQ; ++v)
+            gauss_deriv_field(&o, gmg, S + (size_t)(0 * NEQ + v) * ns, S + (size_t)(1 * NEQ + v) * ns,
+                              S + (size_t)(2 * NEQ + v) * ns, G[v][0], G[v][1], G[v][2]);
+        gauss_deriv_field(&o, gmg, Tst[0], Tst[1], Tst[2], G[5][0], G[5][1], G[5][2]);
+        const double *jq = JQ + (size_t)e * NEQ * np;
+        const double *jac = mesh->jacob + (size_t)e * np;
+        const double c3 = pow((double)(ncg * ncg * ncg), 1.0 / 3.0);
+        for (int pt = 0; pt < np; ++pt) {
+            double Q[NEQ], Qx[NEQ], Qy[NEQ], Qz[NEQ], T1[3], rhs, duc;
+            for (int v = 0; v < NEQ; ++v) {
+                Q[v] = jq[v * np + pt] * jac[pt];
+                Qx[v] = G[v][0][pt];
+                Qy[v] = G[v][1][pt];
+                Qz[v] = G[v][2][pt];
+            }
+            T1[0] = G[5][0][pt]; T1[1] = G[5][1][pt]; T1[2] = G[5][2][pt];
+            ev_terms(ph, Q, Qx, Qy, Qz, T1, &rhs, &duc);
+            if (resid) resid[(size_t)e * np + pt] = rhs;
+            if (ducros) ducros[(size_t)e * np + pt] = duc;
+            const double rho = jq[0 * np + pt] * jac[pt];
+            const double h = pow(fabs(1.0 / jac[pt]), 1.0 / 3.0) / c3;
+            double mur = ev->cmu * ph->re * rho * (h * h) * fabs(rhs);
+            if (ev->ducros) mur = mur * duc;
+            mu[pt] = fmin(fmax(mur, 0.0), ev->mumax);
+        }
+        if (ev->smoothing) ev_smooth(mu, ncg);
+        if (muhg)
+            for (int pt = 0; pt < np; ++pt) muhg[(size_t)e * np + pt] = mu[pt];
+        /* prolong to the three staggered grids: interp -> mxv */
+        for (int d = 0; d < 3; ++d)
+            for (int b = 0; b < ncg; ++b)
+                for (int a = 0; a < ncg; ++a)
+                    for (int j = 0; j < nc; ++j) {
+                        double s = 0.0;
+                        for (int k = 0; k < ncg; ++k) {
+                            double t = o.bx[j * ncg + k] * mu[gidx(d, k, a, b, ncg)];
+                            s = (k == 0) ? t : s + t;
+                        }
+                        muh[((size_t)e * 3 + d) * ns + sidx(d, j, a, b, ncg, nc)] = s;
+                    }
+    }
+    return 0;
+}
+
+int or_rk_step_ev(const or_mesh *mesh, const or_phys *ph, const or_ev *ev, double *JQ, double *G, double dt,
+                  double *stag_lag, int have_lag, double *muh, double *resid, int nthreads)
+{
+    double ark[5], brk[5], crk[5];
+    or_rkcoefs(ark, brk, crk);
+    const int np = (mesh->p + 1) * (mesh->p + 1) * (mesh->p + 1);
+    const int ns = (mesh->p + 2) * (mesh->p + 1) * (mesh->p + 1);
+    const size_t n = (size_t)mesh->K * NEQ * np;
+    if (have_lag) {
+        int rc = or_ev_viscosity(mesh, ph, ev, JQ, stag_lag, NULL, muh, NULL, NULL);
+        if (rc) return rc;
+    } else {
+        memset(muh, 0, sizeof(double) * (size_t)mesh->K * 3 * ns);
+    }
+    double *Qt = (double *)malloc(sizeof(double) * n);
+    if (!Qt) return -2;
+    memset(G, 0, sizeof(double) * n);
+    double last = 0.0;
+    for (int kk = 0; kk < 5; ++kk) {
+        or_debug dbg = {kk == 4 ? stag_lag : NULL, NULL, NULL, NULL};
+        int rc = or_rhs(mesh, ph, JQ, muh, Qt, kk == 4 ? &dbg : NULL, nthreads);
+        if (rc) { free(Qt); return rc; }
+        const double a = ark[kk], c = crk[kk];
+#pragma omp parallel for schedule(static)
+        for (int e = 0; e < mesh->K; ++e)
+            for (int i = 0; i < NEQ * np; ++i) {
+                const size_t g = (size_t)e * NEQ * np + i;
+                G[g] = a * G[g] + Qt[g];
+                JQ[g] = JQ[g] + c * dt * G[g];
+            }
+        last = 0.0;
+        const double *q = Qt + (size_t)(mesh->K - 1) * NEQ * np;
+        for (int i = 0; i < NEQ * np; ++i) last = fmax(fabs(q[i]), last);
+    }
+    if (resid) *resid = last;
+    free(Qt);
+    return 0;
+}

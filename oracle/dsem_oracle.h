@@ -65,6 +65,25 @@ void or_vis_flux(const or_phys *ph, const double *Q, const double *Qx, const dou
 
 int  or_rhs(const or_mesh *mesh, const or_phys *ph, const double *JQ, const double *muh,
             double *Qt, or_debug *dbg, int nthreads);
+
+/* Entropy viscosity (EV_routines/Main/Driver.f90:21-60). */
+typedef struct {
+    double cmu, mumax;
+    int32_t ducros, smoothing;
+} or_ev;
+
+/* mu_h from the state JQ and the lagged staggered states stag_lag
+ * [K][3][5][ns] (the face-averaged Qlgg/Qglg/Qggl of the previous step's
+ * last stage).  Outputs: muhg [K][np] (Gauss points), muh [K][3][ns]
+ * (staggered grids), resid/ducros [K][np] (nullable). */
+int  or_ev_viscosity(const or_mesh *mesh, const or_phys *ph, const or_ev *ev, const double *JQ,
+                     const double *stag_lag, double *muhg, double *muh, double *resid, double *ducros);
+
+/* tstep with entropy viscosity: mu_h from stag_lag (zero when have_lag = 0,
+ * the reference's fresh-start behaviour), five stages, and stag_lag
+ * replaced by the last stage's staggered states. */
+int  or_rk_step_ev(const or_mesh *mesh, const or_phys *ph, const or_ev *ev, double *JQ, double *G, double dt,
+                   double *stag_lag, int have_lag, double *muh, double *resid, int nthreads);
 int  or_rk_step(const or_mesh *mesh, const or_phys *ph, double *JQ, double *G, double dt,
                 const double *muh, double *resid, int nthreads);
 double or_dtcalc(const or_mesh *mesh, const or_phys *ph, const double *JQ);

@@ -32,6 +32,10 @@ class OrPhys(C.Structure):
     _fields_ = [("gamma", C.c_double), ("re", C.c_double), ("pr", C.c_double), ("mach", C.c_double)]
 
 
+class OrEv(C.Structure):
+    _fields_ = [("cmu", C.c_double), ("mumax", C.c_double), ("ducros", C.c_int32), ("smoothing", C.c_int32)]
+
+
 class OrDebug(C.Structure):
     _fields_ = [("stag", _dp), ("flux", _dp), ("grad", _dp), ("visc", _dp)]
 
@@ -67,6 +71,10 @@ def lib():
         L.or_rhs.argtypes = [C.POINTER(OrMesh), C.POINTER(OrPhys), _dp, _dp, _dp, C.POINTER(OrDebug), C.c_int]
         L.or_rk_step.argtypes = [C.POINTER(OrMesh), C.POINTER(OrPhys), _dp, _dp, C.c_double, _dp, _dp, C.c_int]
         L.or_dtcalc.argtypes = [C.POINTER(OrMesh), C.POINTER(OrPhys), _dp]
+        L.or_ev_viscosity.argtypes = [C.POINTER(OrMesh), C.POINTER(OrPhys), C.POINTER(OrEv), _dp, _dp, _dp, _dp,
+                                      _dp, _dp]
+        L.or_rk_step_ev.argtypes = [C.POINTER(OrMesh), C.POINTER(OrPhys), C.POINTER(OrEv), _dp, _dp, C.c_double,
+                                    _dp, C.c_int, _dp, _dp, C.c_int]
         L.or_dtcalc.restype = C.c_double
         _lib = L
     return _lib
@@ -217,3 +225,30 @@ def vis_flux(Q, Qx, Qy, Qz, amu=1.0, ph=None):
     a = [np.ascontiguousarray(t, dtype=np.float64) for t in (Q, Qx, Qy, Qz)]
     lib().or_vis_flux(C.byref(ph), *[ptr(t) for t in a], float(amu), ptr(f), ptr(g), ptr(h))
     return f, g, h
+
+
+# ---------------------------------------------------------------- entropy viscosity
+def ev_viscosity(geom: Geometry, JQ, stag_lag, ev: OrEv, ph: OrPhys | None = None):
+    """(muhg [K,np], muh [K,3,ns], resid [K,np], ducros [K,np])."""
+    ph = ph or phys()
+    K = geom.K
+    muhg, resid, duc = np.zeros((K, geom.np)), np.zeros((K, geom.np)), np.zeros((K, geom.np))
+    muh = np.zeros((K, 3, geom.ns))
+    m = geom.cmesh()
+    rc = lib().or_ev_viscosity(C.byref(m), C.byref(ph), C.byref(ev), ptr(np.ascontiguousarray(JQ)),
+                               ptr(np.ascontiguousarray(stag_lag)), ptr(muhg), ptr(muh), ptr(resid), ptr(duc))
+    assert rc == 0
+    return muhg, muh, resid, duc
+
+
+def rk_step_ev(geom: Geometry, JQ, dt, stag_lag, have_lag, ev: OrEv, ph: OrPhys | None = None, nthreads=0):
+    """One EV step in place; stag_lag [K,3,5,ns] updated in place.  Returns (JQ, muh, resid)."""
+    ph = ph or phys()
+    G = np.zeros_like(JQ)
+    muh = np.zeros((geom.K, 3, geom.ns))
+    res = C.c_double(0.0)
+    m = geom.cmesh()
+    rc = lib().or_rk_step_ev(C.byref(m), C.byref(ph), C.byref(ev), ptr(JQ), ptr(G), dt, ptr(stag_lag),
+                             int(have_lag), ptr(muh), C.byref(res), nthreads)
+    assert rc == 0
+    return JQ, muh, res.value

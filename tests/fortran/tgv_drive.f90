@@ -32,7 +32,7 @@ program tgv_drive
    desc%mortar_sign = c_loc(ms); desc%mortar_nsign = c_loc(mn)
    desc%mortar_orient = c_null_ptr; desc%corners = c_loc(corners)
    prm%gamma = phys(1); prm%re = phys(2); prm%pr = phys(3); prm%mach = phys(4)
-   prm%shock = 0; prm%precision = DSEM_PREC_FP64; prm%metrics = DSEM_METRICS_PER_POINT; prm%reserved = 0
+   prm%shock = 0; prm%precision = DSEM_PREC_FP64; prm%metrics = DSEM_METRICS_PER_POINT; prm%ev_flags = 0
    prm%cmu = 1.0d0; prm%mumax = 1.0d0; prm%cmax = 0.5d0
 
    rc = dsem_create(desc, prm, 0, ctx)
