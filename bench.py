@@ -98,11 +98,24 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-roofline", action="store_true")
+    ap.add_argument("--copy-bytes", type=int, default=0,
+                    help="run one device-to-device copy kernel of this many bytes first (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (multi-GPU sharding is a "
                                    "design note only)"}))
         sys.exit(2)
+
+    if args.copy_bytes > 0:
+        # a plain copy kernel of known size, so that profiler byte counters
+        # (FETCH_SIZE / WRITE_SIZE) can be scaled against it
+        import torch
+        n = args.copy_bytes // 8
+        src = torch.ones(n, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     from dsem_amd.solver import DSEM
     mesh = M.periodic_box(args.elements, args.p)

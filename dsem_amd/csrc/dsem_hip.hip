@@ -108,12 +108,12 @@ __device__ __forceinline__ int face_of(int d, int j)
 
 // ------------------------------------------------------------ pointwise physics
 // Pointwise physics.  Euler fluxes: fflux/gflux/hflux (PhysLib.f90:17-124).
-// Viscous fluxes x/y/z_vis_flux (PhysLib.f90:419-583), split in two steps so
-// that the conservative gradients of one direction are consumed as soon as
-// they are formed: prim_gradient turns dQ/dx_c into (du/dx_c, dv/dx_c,
-// dw/dx_c, dT/dx_c) (PhysLib.f90:448-462) and viscous_from_primitive forms
-// the Stokes stress and Fourier heat flux (PhysLib.f90:464-470).  Both are
-// contracted with one metric row J grad xi_d as Int_fluxes/Vis_Fluxes do.
+// Viscous fluxes x/y/z_vis_flux (PhysLib.f90:419-583), split so that the
+// conservative gradients of one direction are consumed as soon as they are
+// formed: prim_gradient turns dQ/dx_c into (du/dx_c, dv/dx_c, dw/dx_c,
+// dT/dx_c) (PhysLib.f90:448-462); the stress and heat flux are formed in
+// the volume kernel.  Both families are contracted with one metric row
+// J grad xi_d as Int_fluxes/Vis_Fluxes do.
 struct Prim {
     double rho, rinv, u, v, w, q2, p;
 };
@@ -143,30 +143,6 @@ __device__ __forceinline__ void prim_gradient(const DevPhys &ph, const Prim &s, 
     out[1] = vc;
     out[2] = wc;
     out[3] = (pc - s.p * Qc[0] * s.rinv) * ph.tfac * s.rinv;
-}
-
-// L[i][c] = d u_i / d x_c (i = u,v,w), T[c] = dT/dx_c.  Kept out of line:
-// when inlined, the compiler interleaves it with the caller's prolongation
-// loads and runs out of registers; the call boundary (arguments through the
-// stack) keeps the kernel at about 150 VGPRs.
-__device__ __attribute__((noinline)) void viscous_from_primitive(const DevPhys &ph, const Prim &s, const double L[3][3],
-                                                                 const double *T, double amu, double g0, double g1,
-                                                                 double g2, double *r)
-{
-    const double hfac = (1.0 + (amu - 1.0) * ph.kap) * ph.hfac1;
-    const double third = 1.0 / 3.0;
-    const double div = L[0][0] + L[1][1] + L[2][2];
-    const double txx = 2.0 * amu * (L[0][0] - div * third), tyy = 2.0 * amu * (L[1][1] - div * third);
-    const double tzz = 2.0 * amu * (L[2][2] - div * third);
-    const double txy = amu * (L[0][1] + L[1][0]), txz = amu * (L[2][0] + L[0][2]), tyz = amu * (L[2][1] + L[1][2]);
-    const double f4 = s.u * txx + s.v * txy + s.w * txz + hfac * T[0];
-    const double g4 = s.v * tyy + s.u * txy + s.w * tyz + hfac * T[1];
-    const double h4 = s.v * tyz + s.u * txz + s.w * tzz + hfac * T[2];
-    r[0] = 0.0;
-    r[1] = g0 * txx + g1 * txy + g2 * txz;
-    r[2] = g0 * txy + g1 * tyy + g2 * tyz;
-    r[3] = g0 * txz + g1 * tyz + g2 * tzz;
-    r[4] = g0 * f4 + g1 * g4 + g2 * h4;
 }
 
 // Euler flux contracted with one metric row, from primitives.
@@ -343,9 +319,13 @@ __device__ __forceinline__ void volume_grid(const DevMesh &m, const DevPhys &ph,
             for (int v = 0; v < 5; ++v) Slag[(((size_t)e * 3 + D) * NEQ + v) * S::NS + s] = Qv[v];
         }
         const Prim pr = primitives(ph, Qv);
-        __builtin_amdgcn_sched_barrier(0);
-        double Lg[3][3], Tg[3];
-#pragma unroll
+        // Velocity and temperature gradients, one direction c at a time: the
+        // prolonged conservative gradient dQ/dx_c (5 fields) is turned into
+        // (du/dx_c, dv/dx_c, dw/dx_c, dT/dx_c) and folded into the symmetric
+        // velocity-gradient sums, so no more than one direction's loads are live.
+        double Sxx = 0.0, Syy = 0.0, Szz = 0.0, Sxy = 0.0, Sxz = 0.0, Syz = 0.0, div = 0.0;
+        double Tx = 0.0, Ty = 0.0, Tz = 0.0;
+#pragma unroll 1
         for (int c = 0; c < 3; ++c) {
             double Gc[5];
 #pragma unroll
@@ -355,22 +335,41 @@ __device__ __forceinline__ void volume_grid(const DevMesh &m, const DevPhys &ph,
 #pragma unroll
                 for (int k = 0; k < S::NCG; ++k) t = fma(Bj[k], gl[gidx<P>(D, k, a, b)], t);
                 Gc[v] = t;
-                if (v == 1 || v == 3) __builtin_amdgcn_sched_barrier(0);
             }
             double o[4];
             prim_gradient(ph, pr, Gc, o);
-            Lg[0][c] = o[0];
-            Lg[1][c] = o[1];
-            Lg[2][c] = o[2];
-            Tg[c] = o[3];
-            __builtin_amdgcn_sched_barrier(0);
+            const double uc = o[0], vc = o[1], wc = o[2];
+            div = div + (c == 0 ? uc : (c == 1 ? vc : wc));
+            Sxx += (c == 0) ? uc : 0.0;
+            Syy += (c == 1) ? vc : 0.0;
+            Szz += (c == 2) ? wc : 0.0;
+            Sxy += (c == 0) ? vc : ((c == 1) ? uc : 0.0);   // uy + vx
+            Sxz += (c == 0) ? wc : ((c == 2) ? uc : 0.0);   // wx + uz
+            Syz += (c == 1) ? wc : ((c == 2) ? vc : 0.0);   // wy + vz
+            Tx = (c == 0) ? o[3] : Tx;
+            Ty = (c == 1) ? o[3] : Ty;
+            Tz = (c == 2) ? o[3] : Tz;
         }
         double amu = 1.0;
         if (muh) amu = amu + muh[((size_t)e * 3 + D) * S::NS + s];
         const int sq = s * m.ms;
         const double g0 = gmd[0 * nsm + sq], g1 = gmd[1 * nsm + sq], g2 = gmd[2 * nsm + sq];
         double rv[5];
-        viscous_from_primitive(ph, pr, Lg, Tg, amu, g0, g1, g2, rv);
+        {
+            // Stokes stress and Fourier heat flux (x/y/z_vis_flux, PhysLib.f90:464-470)
+            const double hfac = (1.0 + (amu - 1.0) * ph.kap) * ph.hfac1;
+            const double d3 = div * (1.0 / 3.0);
+            const double txx = 2.0 * amu * (Sxx - d3), tyy = 2.0 * amu * (Syy - d3), tzz = 2.0 * amu * (Szz - d3);
+            const double txy = amu * Sxy, txz = amu * Sxz, tyz = amu * Syz;
+            const double f4 = pr.u * txx + pr.v * txy + pr.w * txz + hfac * Tx;
+            const double g4 = pr.v * tyy + pr.u * txy + pr.w * tyz + hfac * Ty;
+            const double h4 = pr.v * tyz + pr.u * txz + pr.w * tzz + hfac * Tz;
+            rv[0] = 0.0;
+            rv[1] = g0 * txx + g1 * txy + g2 * txz;
+            rv[2] = g0 * txy + g1 * tyy + g2 * tyz;
+            rv[3] = g0 * txz + g1 * tyz + g2 * tzz;
+            rv[4] = g0 * f4 + g1 * g4 + g2 * h4;
+        }
         if (face) {
             const double nrm = m.norm[(size_t)mo * nfm + q * m.ms];
 #pragma unroll
