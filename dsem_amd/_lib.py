@@ -69,6 +69,8 @@ def load() -> C.CDLL:
         "dsem_ev_update": ([vp, _dp, _dp, _dp], C.c_int),
         "dsem_set_lagged_states": ([vp, _dp], C.c_int),
         "dsem_get_lagged_states": ([vp, _dp], C.c_int),
+        "dsem_reduce_ke": ([vp, _dp, _dp], C.c_int),
+        "dsem_keep_lagged_states": ([vp, C.c_int], C.c_int),
         "dsem_rhs": ([vp, C.c_double, _dp], C.c_int),
         "dsem_rk_step": ([vp, C.c_double, C.c_double, _dp], C.c_int),
         "dsem_rk_steps": ([vp, C.c_double, C.c_double, C.c_int, _dp], C.c_int),

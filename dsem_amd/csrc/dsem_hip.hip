@@ -796,6 +796,109 @@ __global__ __launch_bounds__(BLOCK) void k_ev(const double *__restrict__ JQ, con
     }
 }
 
+// Per-element partial sums of the flow diagnostics, part[e][0..4]:
+//   0..2  sum (u^2, v^2, w^2) bmat over the Gauss points, u = (JQ)_2/(JQ)_1
+//         (turbulent kinetic energy of do_user_stuff,
+//         caseFiles/tgv/BackstepProblem-stoch.f90:629-650)
+//   3, 4  sum |omega|^2/jacob and sum 1/jacob (energydissipation,
+//         src/Energydissipation/energydissipation.f90:19-85), with the
+//         velocity gradients from Gauss_Deriv of the lagged staggered states
+//         Slag (fields rho, rho u, rho v, rho w); skipped when Slag is null.
+template <int P>
+__global__ __launch_bounds__(BLOCK) void k_diag(const double *__restrict__ JQ, const double *__restrict__ Slag,
+                                                 DevMesh m, const double *__restrict__ bmat, double *__restrict__ part)
+{
+    using S = Sz<P>;
+    __shared__ double sS[3 * 4 * S::NS];
+    __shared__ double sRed[5][BLOCK];
+    const int e = blockIdx.x, tid = threadIdx.x;
+    const bool vort = Slag != nullptr;
+    if (vort) {
+        const double *src = Slag + (size_t)e * 3 * NEQ * S::NS;
+        for (int i = tid; i < 3 * 4 * S::NS; i += BLOCK) {
+            const int d = i / (4 * S::NS), r = i % (4 * S::NS);
+            sS[i] = src[d * NEQ * S::NS + r];
+        }
+    }
+    __syncthreads();
+    const int npm = m.ms ? S::NP : 1;
+    const double *gg = m.gmetg + (size_t)e * 9 * npm;
+    const double *jq = JQ + (size_t)e * NEQ * S::NP;
+    const double *jac = m.jacob + (size_t)e * npm;
+    double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int pt = tid; pt < S::NP; pt += BLOCK) {
+        const double q1 = jq[pt];
+        const double u = jq[S::NP + pt] / q1, v = jq[2 * S::NP + pt] / q1, w = jq[3 * S::NP + pt] / q1;
+        const double bw = bmat[pt];
+        acc[0] += (u * u) * bw;
+        acc[1] += (v * v) * bw;
+        acc[2] += (w * w) * bw;
+        if (vort) {
+            const int n = pt % S::NCG, mi = (pt / S::NCG) % S::NCG, l = pt / (S::NCG * S::NCG);
+            const int pq = pt * m.ms;
+            double gr[4][3];
+#pragma unroll
+            for (int f = 0; f < 4; ++f) {
+                double wd[3];
+#pragma unroll
+                for (int d = 0; d < 3; ++d) {
+                    const int k = d == 0 ? n : (d == 1 ? mi : l);
+                    const int a = d == 0 ? mi : n;
+                    const int b = d == 2 ? mi : l;
+                    const double *fld = sS + (d * 4 + f) * S::NS;
+                    double t = 0.0;
+#pragma unroll
+                    for (int j = 0; j < S::NC; ++j) t = fma(c_D[P][k * S::NC + j], fld[sidx<P>(d, j, a, b)], t);
+                    wd[d] = t;
+                }
+#pragma unroll
+                for (int c = 0; c < 3; ++c)
+                    gr[f][c] = gg[(0 * 3 + c) * npm + pq] * wd[0] + gg[(1 * 3 + c) * npm + pq] * wd[1] +
+                               gg[(2 * 3 + c) * npm + pq] * wd[2];
+            }
+            const double udery = (gr[1][1] - u * gr[0][1]) / q1, uderz = (gr[1][2] - u * gr[0][2]) / q1;
+            const double vderx = (gr[2][0] - v * gr[0][0]) / q1, vderz = (gr[2][2] - v * gr[0][2]) / q1;
+            const double wderx = (gr[3][0] - w * gr[0][0]) / q1, wdery = (gr[3][1] - w * gr[0][1]) / q1;
+            const double jc = jac[pq];
+            const double c1 = (wdery - vderz) / jc, c2 = (uderz - wderx) / jc, c3 = (vderx - udery) / jc;
+            acc[3] += (c1 * c1 + c2 * c2 + c3 * c3) / jc;
+            acc[4] += 1.0 / jc;
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < 5; ++i) sRed[i][tid] = acc[i];
+    __syncthreads();
+    for (int st = BLOCK / 2; st > 0; st >>= 1) {
+        if (tid < st)
+#pragma unroll
+            for (int i = 0; i < 5; ++i) sRed[i][tid] += sRed[i][tid + st];
+        __syncthreads();
+    }
+    if (tid < 5) part[(size_t)e * 5 + tid] = sRed[tid][0];
+}
+
+// Sum of the per-element partials in a fixed order (one workgroup), so the
+// diagnostics are reproducible from run to run.
+__global__ __launch_bounds__(BLOCK) void k_diag_sum(const double *__restrict__ part, int K, double *__restrict__ out)
+{
+    __shared__ double sRed[5][BLOCK];
+    const int tid = threadIdx.x;
+    double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int e = tid; e < K; e += BLOCK)
+#pragma unroll
+        for (int i = 0; i < 5; ++i) acc[i] += part[(size_t)e * 5 + i];
+#pragma unroll
+    for (int i = 0; i < 5; ++i) sRed[i][tid] = acc[i];
+    __syncthreads();
+    for (int st = BLOCK / 2; st > 0; st >>= 1) {
+        if (tid < st)
+#pragma unroll
+            for (int i = 0; i < 5; ++i) sRed[i][tid] += sRed[i][tid + st];
+        __syncthreads();
+    }
+    if (tid < 5) out[tid] = sRed[tid][0];
+}
+
 // dtcalc (dtcalc.f90:9-58): per point 1/dti, reduced to the minimum
 template <int P>
 __global__ __launch_bounds__(BLOCK) void k_dtcalc(const double *__restrict__ JQ, DevMesh m, DevPhys ph,
@@ -834,9 +937,9 @@ __global__ __launch_bounds__(BLOCK) void k_dtcalc(const double *__restrict__ JQ,
 // ====================================================================== host
 namespace {
 
-enum KernelId { KID_TRACES = 0, KID_FACE_INV, KID_VOLUME, KID_FACE_VISC, KID_UPDATE, KID_DTCALC, KID_EV, KID_COUNT };
+enum KernelId { KID_TRACES = 0, KID_FACE_INV, KID_VOLUME, KID_FACE_VISC, KID_UPDATE, KID_DTCALC, KID_EV, KID_DIAG, KID_COUNT };
 const char *const kKernelNames[KID_COUNT] = {"k_traces", "k_face_inv", "k_volume", "k_face_visc",
-                                             "k_update", "k_dtcalc", "k_ev"};
+                                             "k_update", "k_dtcalc", "k_ev", "k_diag"};
 
 template <typename T>
 struct DevBuf {
@@ -862,6 +965,8 @@ struct dsem_ctx {
     double *nhat = nullptr, *norm = nullptr, *muh = nullptr;
     double *Slag = nullptr, *muhg = nullptr, *ev_resid = nullptr, *ev_ducros = nullptr;   // entropy viscosity
     bool have_lag = false;
+    bool keep_lag = false;                     // save the last stage's staggered states without EV
+    double *bmat = nullptr, *diag_part = nullptr, *diag_out = nullptr;   // diagnostics
     int *e2m = nullptr, *m_elem = nullptr, *m_face = nullptr, *m_sign = nullptr, *m_nsign = nullptr;
     unsigned long long *d_scalars = nullptr;   // [0] resid bits, [1] dt bits, [2] nonfinite flag
     std::vector<void *> allocations;
@@ -1053,6 +1158,22 @@ int dtcalc_dispatch(dsem_ctx *c, unsigned long long *out)
     DISPATCH_P(c, run_dtcalc, out);
 }
 
+template <int P>
+int run_diag(dsem_ctx *c, bool vorticity)
+{
+    const DevMesh m = dev_mesh(c);
+    const double *slag = vorticity ? c->Slag : nullptr;
+    return launch(c, KID_DIAG, [&] {
+        k_diag<P><<<c->K, BLOCK, 0, c->stream>>>(c->JQ, slag, m, c->bmat, c->diag_part);
+        k_diag_sum<<<1, BLOCK, 0, c->stream>>>(c->diag_part, c->K, c->diag_out);
+    });
+}
+
+int diag_dispatch(dsem_ctx *c, bool vorticity)
+{
+    DISPATCH_P(c, run_diag, vorticity);
+}
+
 int upload_operators(dsem_ctx *c)
 {
     double D[9][90] = {}, B[9][90] = {}, Mo[9][81] = {}, cx[9][10] = {};
@@ -1161,6 +1282,9 @@ int create_impl(const dsem_mesh_desc *mesh, const dsem_params *prm, int device, 
     TRY(dev_alloc(c, &c->FT, (size_t)c->M * NEQ * c->nf));
     TRY(dev_alloc(c, &c->muh, (size_t)c->K * 3 * c->ns));
     TRY(dev_alloc(c, &c->muhg, (size_t)c->K * c->np));
+    TRY(upload(c, &c->bmat, dsem::ke_weights(c->ops)));
+    TRY(dev_alloc(c, &c->diag_part, (size_t)c->K * 5));
+    TRY(dev_alloc(c, &c->diag_out, 8));
     if (c->params.shock) {
         TRY(dev_alloc(c, &c->Slag, (size_t)c->K * 3 * NEQ * c->ns));
         TRY(dev_alloc(c, &c->ev_resid, (size_t)c->K * c->np));
@@ -1318,7 +1442,7 @@ int dsem_ev_update(dsem_ctx *c, double *muhg, double *resid, double *ducros)
 
 int dsem_set_lagged_states(dsem_ctx *c, const double *stag)
 {
-    if (!c || !c->stream || !stag || !c->Slag) return set_err(c, DSEM_ERR_ARG, "set_lagged_states needs shock = 1");
+    if (!c || !c->stream || !stag || !c->Slag) return set_err(c, DSEM_ERR_ARG, "lagged states need shock = 1 or dsem_keep_lagged_states");
     HIP_TRY(c, hipSetDevice(c->device));
     HIP_TRY(c, hipMemcpyAsync(c->Slag, stag, sizeof(double) * (size_t)c->K * 3 * NEQ * c->ns, hipMemcpyHostToDevice,
                               c->stream));
@@ -1329,11 +1453,40 @@ int dsem_set_lagged_states(dsem_ctx *c, const double *stag)
 
 int dsem_get_lagged_states(dsem_ctx *c, double *stag)
 {
-    if (!c || !c->stream || !stag || !c->Slag) return set_err(c, DSEM_ERR_ARG, "get_lagged_states needs shock = 1");
+    if (!c || !c->stream || !stag || !c->Slag) return set_err(c, DSEM_ERR_ARG, "lagged states need shock = 1 or dsem_keep_lagged_states");
     HIP_TRY(c, hipSetDevice(c->device));
     HIP_TRY(c, hipMemcpyAsync(stag, c->Slag, sizeof(double) * (size_t)c->K * 3 * NEQ * c->ns, hipMemcpyDeviceToHost,
                               c->stream));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return DSEM_OK;
+}
+
+int dsem_keep_lagged_states(dsem_ctx *c, int enable)
+{
+    if (!c || !c->stream) return set_err(c, DSEM_ERR_ARG, "keep_lagged_states arguments");
+    HIP_TRY(c, hipSetDevice(c->device));
+    if (enable && !c->Slag) TRY(dev_alloc(c, &c->Slag, (size_t)c->K * 3 * NEQ * c->ns));
+    c->keep_lag = enable != 0;
+    return DSEM_OK;
+}
+
+int dsem_reduce_ke(dsem_ctx *c, double *Ek, double *eps)
+{
+    if (!c || !c->stream || (!Ek && !eps)) return set_err(c, DSEM_ERR_ARG, "reduce_ke arguments");
+    if (eps && !(c->Slag && c->have_lag))
+        return set_err(c, DSEM_ERR_ARG, "the enstrophy dissipation needs the lagged staggered states of a step taken "
+                                        "with shock = 1 or dsem_keep_lagged_states");
+    HIP_TRY(c, hipSetDevice(c->device));
+    TRY(diag_dispatch(c, eps != nullptr));
+    double h[5];
+    HIP_TRY(c, hipMemcpyAsync(h, c->diag_out, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    const double pi = 3.141592653589793238462643;
+    if (Ek) {
+        const double j0 = c->h_jacob[0];
+        *Ek = 0.5 * (h[0] / j0 + h[1] / j0 + h[2] / j0) / (8.0 * (pi * pi * pi));
+    }
+    if (eps) *eps = 100.0 * (h[3] / h[4]) * (1.0 / c->params.re);
     return DSEM_OK;
 }
 
@@ -1382,9 +1535,9 @@ int dsem_rk_steps(dsem_ctx *c, double t, double dt, int nsteps, double *resid_ou
             st.nonfinite = check ? reinterpret_cast<unsigned int *>(&c->d_scalars[2]) : nullptr;
             const bool ready = !(s == 0 && k == 0);
             const bool make = !(s == nsteps - 1 && k == 4);
-            TRY(rhs_dispatch(c, c->JQ, st, nullptr, ready, make, c->params.shock && k == 4));
+            TRY(rhs_dispatch(c, c->JQ, st, nullptr, ready, make, (c->params.shock || c->keep_lag) && k == 4));
         }
-        if (c->params.shock) c->have_lag = true;
+        if (c->params.shock || c->keep_lag) c->have_lag = true;
     }
     if (check) {
         unsigned long long h[4];

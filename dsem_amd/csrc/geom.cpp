@@ -222,4 +222,60 @@ void mortar_geometry(const Ops &o, int M, const double *corners, const int32_t *
     }
 }
 
+// Gauss-Legendre rule on [-1,1] (ascending nodes) by Newton's method on the
+// Legendre recurrence; the reference takes the same rule from gaussq
+// (src/Independentroutines/legendre_routines.f90:98-150).
+static void gauss_legendre(int n, double *t, double *w)
+{
+    const double pi = 3.141592653589793238462643;
+    auto legendre = [n](double x, double &pn, double &dpn) {
+        double p0 = 1.0, p1 = x;
+        for (int k = 2; k <= n; ++k) {
+            const double p2 = ((2.0 * k - 1.0) * x * p1 - (k - 1.0) * p0) / k;
+            p0 = p1;
+            p1 = p2;
+        }
+        pn = p1;
+        dpn = (n == 1) ? 1.0 : n * (x * p1 - p0) / (x * x - 1.0);
+    };
+    for (int i = 0; i < n; ++i) {
+        double x = std::cos(pi * (i + 0.75) / (n + 0.5)), pn, dpn;
+        for (int it = 0; it < 100; ++it) {
+            legendre(x, pn, dpn);
+            const double dx = pn / dpn;
+            x -= dx;
+            if (std::fabs(dx) < 1e-16) break;
+        }
+        legendre(x, pn, dpn);
+        t[n - 1 - i] = x;
+        w[n - 1 - i] = 2.0 / ((1.0 - x * x) * dpn * dpn);
+    }
+}
+
+// bmatu1b (caseFiles/tgv/BackstepProblem-stoch.f90:345-361): the triple
+// Gauss-Legendre sum of products of the Chebyshev-Gauss Lagrange bases at
+// x = (t+1)/2, in the reference's summation order.
+std::vector<double> ke_weights(const Ops &o)
+{
+    const int n = o.ncg;
+    double t[16], w[16];
+    gauss_legendre(n, t, w);
+    std::vector<double> b((size_t)n * n * n);
+    for (int i = 1; i <= n; ++i)
+        for (int j = 1; j <= n; ++j)
+            for (int l = 1; l <= n; ++l) {
+                double sum = 0.0;
+                for (int k = 1; k <= n; ++k)
+                    for (int m = 1; m <= n; ++m)
+                        for (int q = 1; q <= n; ++q) {
+                            const double hl = lagrange(l, 0.5 * (t[q - 1] + 1.0), n, o.cxg);
+                            const double hj = lagrange(j, 0.5 * (t[m - 1] + 1.0), n, o.cxg);
+                            const double hi = lagrange(i, 0.5 * (t[k - 1] + 1.0), n, o.cxg);
+                            sum = sum + 0.1250 * w[k - 1] * w[m - 1] * w[q - 1] * hi * hj * hl;
+                        }
+                b[(i - 1) + n * ((j - 1) + n * (l - 1))] = sum;
+            }
+    return b;
+}
+
 }  // namespace dsem

@@ -33,4 +33,8 @@ bool element_geometry(const Ops &o, int K, const double *corners, Geometry &g);
 void mortar_geometry(const Ops &o, int M, const double *corners, const int32_t *m_elem,
                      const int32_t *m_face, std::vector<double> &nhat, std::vector<double> &norm);
 
+// Tensor quadrature weights of the Gauss-point interpolant on [0,1]^3
+// (bmatu1b of the TGV statistics), [ncg^3], index i + ncg*(j + ncg*l).
+std::vector<double> ke_weights(const Ops &o);
+
 }  // namespace dsem

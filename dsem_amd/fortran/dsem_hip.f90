@@ -98,6 +98,19 @@ module dsem_hip
          type(c_ptr), value :: ctx
          real(c_double), intent(in) :: muh(*)
       end function dsem_set_viscosity
+
+      ! turbk of do_user_stuff and eps of energydissipation (either may be
+      ! c_null_ptr through the c_loc of an absent target; see dsem_energy)
+      integer(c_int) function dsem_reduce_ke(ctx, Ek, eps) bind(C, name='dsem_reduce_ke')
+         import :: c_int, c_ptr
+         type(c_ptr), value :: ctx, Ek, eps
+      end function dsem_reduce_ke
+
+      integer(c_int) function dsem_keep_lagged_states(ctx, enable) bind(C, name='dsem_keep_lagged_states')
+         import :: c_int, c_ptr
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: enable
+      end function dsem_keep_lagged_states
    end interface
 
 contains
@@ -128,5 +141,16 @@ contains
       integer(c_int), intent(out) :: ierr
       ierr = dsem_rk_step(ctx, time, dt(1), resid)
    end subroutine dsem_tstep
+
+   ! Statistics of the TGV driver after a step: turbk (fort.22) and the
+   ! enstrophy dissipation of energydissipation (the `dissipated` file);
+   ! the caller forms dissp = (prestep - turbk)/dt as do_user_stuff does.
+   ! eps needs dsem_keep_lagged_states(ctx, 1) before the step.
+   subroutine dsem_energy(ctx, turbk, eps, ierr)
+      type(c_ptr), intent(in) :: ctx
+      real(c_double), intent(out), target :: turbk, eps
+      integer(c_int), intent(out) :: ierr
+      ierr = dsem_reduce_ke(ctx, c_loc(turbk), c_loc(eps))
+   end subroutine dsem_energy
 
 end module dsem_hip

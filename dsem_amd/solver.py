@@ -144,6 +144,25 @@ class DSEM:
         self._check(self._L.dsem_get_lagged_states(self._h, _dp(out)))
         return out
 
+    # ------------------------------------------------------------ diagnostics
+    def keep_lagged_states(self, enable: bool = True) -> None:
+        """Keep the last stage's staggered states after every step (needed by
+        the enstrophy diagnostic when entropy viscosity is off)."""
+        self._check(self._L.dsem_keep_lagged_states(self._h, int(enable)))
+
+    def kinetic_energy(self) -> float:
+        """turbk of the TGV statistics: 1/2 <u^2+v^2+w^2> over the (2 pi)^3 box."""
+        ek = C.c_double(0.0)
+        self._check(self._L.dsem_reduce_ke(self._h, C.byref(ek), None))
+        return ek.value
+
+    def diagnostics(self) -> tuple[float, float]:
+        """(Ek, eps): kinetic energy and the enstrophy dissipation of
+        energydissipation.f90 (needs lagged staggered states)."""
+        ek, eps = C.c_double(0.0), C.c_double(0.0)
+        self._check(self._L.dsem_reduce_ke(self._h, C.byref(ek), C.byref(eps)))
+        return ek.value, eps.value
+
     # ------------------------------------------------------------------ path
     def rhs(self, t: float = 0.0) -> np.ndarray:
         out = np.zeros((self.K, 5, self.np))

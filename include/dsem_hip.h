@@ -143,6 +143,26 @@ int  dsem_rk_steps(dsem_ctx *ctx, double t, double dt, int nsteps, double *resid
 /* Inviscid time-step bound before the CFL factor (dtcalc over all elements). */
 int  dsem_dtcalc(dsem_ctx *ctx, double *dt_out);
 
+/* FP64 flow diagnostics on the device.
+ * Ek  (nullable): turbk of the TGV statistics (do_user_stuff,
+ *     caseFiles/tgv/BackstepProblem-stoch.f90:629-679):
+ *     1/2 sum (u^2+v^2+w^2) bmatu1b / jacob_1(1,1,1) / (8 pi^3), u = (JQ)_2/(JQ)_1;
+ *     the driver's -dEk/dt is (Ek_previous - Ek)/dt (:692).
+ * eps (nullable): energydissipation (src/Energydissipation/energydissipation.f90:2-89):
+ *     100 * sum(|curl u|^2/jacob) / sum(1/jacob) / Re over all Gauss points,
+ *     the velocity gradients taken from the lagged staggered states of the
+ *     last RK stage (as the driver calls it right after tstep,
+ *     src/3DDriver_parfmdf.f90:466-471).  Needs a step taken with shock = 1
+ *     or after dsem_keep_lagged_states(ctx, 1).
+ * Sums run per element, then over elements in a fixed order. */
+int  dsem_reduce_ke(dsem_ctx *ctx, double *Ek, double *eps);
+
+/* Keep the last stage's face-averaged staggered states after every step
+ * (d%Qlgg/Qglg/Qggl survive tstep in the reference) when entropy viscosity
+ * is off, for the enstrophy diagnostic; costs one extra write of
+ * 3*5*nc*ncg^2 doubles per element per step. */
+int  dsem_keep_lagged_states(dsem_ctx *ctx, int enable);
+
 /* Device-pointer entry points (same HIP device, stream-ordered on the
  * context's stream): dJQ and dQt are [K][5][ncg^3] FP64 device buffers. */
 int  dsem_rhs_device(dsem_ctx *ctx, const double *dJQ, double *dQt);

@@ -722,9 +722,11 @@ int or_rhs(const or_mesh *mesh, const or_phys *ph, const double *JQ, const doubl
  * register update (updatea.f90:213-227).  G is zeroed at the start of the
  * step.  resid follows the reference: update_interior resets it per element,
  * so the value left is max|Qt| over the last element of the last stage.
+ * stag_out (nullable, [K][3][5][ns]) receives the last stage's face-averaged
+ * staggered states, which the reference leaves in d%Qlgg/Qglg/Qggl.
  */
 int or_rk_step(const or_mesh *mesh, const or_phys *ph, double *JQ, double *G, double dt,
-               const double *muh, double *resid, int nthreads)
+               const double *muh, double *stag_out, double *resid, int nthreads)
 {
     double ark[5], brk[5], crk[5];
     or_rkcoefs(ark, brk, crk);
@@ -735,7 +737,8 @@ int or_rk_step(const or_mesh *mesh, const or_phys *ph, double *JQ, double *G, do
     memset(G, 0, sizeof(double) * n);
     double last = 0.0;
     for (int kk = 0; kk < 5; ++kk) {
-        int rc = or_rhs(mesh, ph, JQ, muh, Qt, NULL, nthreads);
+        or_debug dbg = {stag_out, NULL, NULL, NULL};
+        int rc = or_rhs(mesh, ph, JQ, muh, Qt, (kk == 4 && stag_out) ? &dbg : NULL, nthreads);
         if (rc) { free(Qt); return rc; }
         const double a = ark[kk], c = crk[kk];
 #pragma omp parallel for schedule(static)
@@ -1021,5 +1024,162 @@ int or_rk_step_ev(const or_mesh *mesh, const or_phys *ph, const or_ev *ev, doubl
     }
     if (resid) *resid = last;
     free(Qt);
+    return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* Kinetic energy and enstrophy-dissipation diagnostics               */
+/* ------------------------------------------------------------------ */
+/* Gauss-Legendre rule on [-1,1], nodes ascending: Newton iteration on the
+ * three-term recurrence of P_n (textbook).  The reference obtains the same
+ * rule from gaussq (src/Independentroutines/legendre_routines.f90:98-150,
+ * Golub-Welsch); both agree to rounding. */
+void or_gauss_legendre(int n, double *t, double *w)
+{
+    for (int i = 0; i < n; ++i) {
+        double x = cos(PI_REF * (i + 0.75) / (n + 0.5)), p0 = 1.0, p1 = x, dp = 1.0;
+        for (int it = 0; it < 100; ++it) {
+            p0 = 1.0;
+            p1 = x;
+            for (int k = 2; k <= n; ++k) {
+                const double p2 = ((2.0 * k - 1.0) * x * p1 - (k - 1.0) * p0) / k;
+                p0 = p1;
+                p1 = p2;
+            }
+            dp = (n == 1) ? 1.0 : n * (x * p1 - p0) / (x * x - 1.0);
+            const double dx = p1 / dp;
+            x -= dx;
+            if (fabs(dx) < 1e-16) break;
+        }
+        p0 = 1.0;
+        p1 = x;
+        for (int k = 2; k <= n; ++k) {
+            const double p2 = ((2.0 * k - 1.0) * x * p1 - (k - 1.0) * p0) / k;
+            p0 = p1;
+            p1 = p2;
+        }
+        dp = (n == 1) ? 1.0 : n * (x * p1 - p0) / (x * x - 1.0);
+        t[n - 1 - i] = x;
+        w[n - 1 - i] = 2.0 / ((1.0 - x * x) * dp * dp);
+    }
+}
+
+/* bmatu1b (caseFiles/tgv/BackstepProblem-stoch.f90:345-361): the tensor
+ * quadrature weights of the Chebyshev-Gauss interpolant on [0,1]^3,
+ * bmat(i,j,l) = sum_{k,m,n} 1/8 w_k w_m w_n h_i(x_k) h_j(x_m) h_l(x_n),
+ * x = (t+1)/2 over the ncg-point Gauss-Legendre rule; index i + ncg(j + ncg l). */
+int or_ke_weights(int p, double *bmat)
+{
+    ops_t o;
+    if (make_ops(p, &o)) return -1;
+    const int ncg = o.ncg;
+    double t[16], w[16];
+    or_gauss_legendre(ncg, t, w);
+    for (int i = 1; i <= ncg; ++i)
+        for (int j = 1; j <= ncg; ++j)
+            for (int l = 1; l <= ncg; ++l) {
+                double sum = 0.0;
+                for (int k = 1; k <= ncg; ++k)
+                    for (int m = 1; m <= ncg; ++m)
+                        for (int n = 1; n <= ncg; ++n) {
+                            const double hl = polyn(l, 0.5 * (t[n - 1] + 1.0), ncg, o.cxg);
+                            const double hj = polyn(j, 0.5 * (t[m - 1] + 1.0), ncg, o.cxg);
+                            const double hi = polyn(i, 0.5 * (t[k - 1] + 1.0), ncg, o.cxg);
+                            sum = sum + 0.1250 * w[k - 1] * w[m - 1] * w[n - 1] * hi * hj * hl;
+                        }
+                bmat[(i - 1) + ncg * ((j - 1) + ncg * (l - 1))] = sum;
+            }
+    return 0;
+}
+
+/* Turbulent kinetic energy of do_user_stuff
+ * (caseFiles/tgv/BackstepProblem-stoch.f90:629-650,653-657,679):
+ * turbk = 1/2 sum_e sum_ijl (u^2 + v^2 + w^2) bmat(i,j,l) / jacob_1(1,1,1) / (8 pi^3),
+ * u = (JQ)_2/(JQ)_1.  The per-point Jacobian is that of the first element's
+ * first point, as in the reference (exact on a uniform mesh).  Loop order
+ * e, i, j, l (l innermost), one running sum per component. */
+int or_kinetic_energy(const or_mesh *mesh, const double *JQ, double *Ek)
+{
+    const int ncg = mesh->p + 1, np = ncg * ncg * ncg;
+    double bmat[729];
+    if (or_ke_weights(mesh->p, bmat)) return -1;
+    double ur = 0.0, vr = 0.0, wr = 0.0;
+    for (int e = 0; e < mesh->K; ++e) {
+        const double *q = JQ + (size_t)e * NEQ * np;
+        for (int i = 0; i < ncg; ++i)
+            for (int j = 0; j < ncg; ++j)
+                for (int l = 0; l < ncg; ++l) {
+                    const int pt = i + ncg * (j + ncg * l);
+                    const double u = q[1 * np + pt] / q[0 * np + pt];
+                    const double v = q[2 * np + pt] / q[0 * np + pt];
+                    const double w = q[3 * np + pt] / q[0 * np + pt];
+                    ur = ur + (u * u) * bmat[pt];
+                    vr = vr + (v * v) * bmat[pt];
+                    wr = wr + (w * w) * bmat[pt];
+                }
+    }
+    const double j0 = mesh->jacob[0];
+    ur = ur / j0;
+    vr = vr / j0;
+    wr = wr / j0;
+    *Ek = 0.5 * (ur + vr + wr) / (8.0 * (PI_REF * PI_REF * PI_REF));
+    return 0;
+}
+
+/* energydissipation (src/Energydissipation/energydissipation.f90:2-89):
+ * eps = 100 (sum |omega|^2 / jacob) / (sum 1/jacob) / Re over every Gauss
+ * point (unweighted), with the velocity gradients from Gauss_Deriv of the
+ * lagged staggered states stag_lag [K][3][5][ns] and u = (JQ)_2/(JQ)_1 of the
+ * current state:  du/dy = (Qy_2 - u Qy_1)/(JQ)_1, omega = curl/jacob.
+ * Loop order e, l, m, n over Q(l,m,n) (n innermost). */
+int or_enstrophy(const or_mesh *mesh, const or_phys *ph, const double *JQ, const double *stag_lag, double *eps)
+{
+    ops_t o;
+    if (make_ops(mesh->p, &o)) return -1;
+    const int K = mesh->K, ncg = o.ncg, np = o.np, ns = o.ns;
+    double *part = (double *)malloc(sizeof(double) * 2 * (size_t)K);
+    if (!part) return -2;
+#pragma omp parallel for schedule(static)
+    for (int e = 0; e < K; ++e) {
+        double G[4][3][729];
+        const double *S = stag_lag + (size_t)e * 3 * NEQ * ns;
+        const double *gmg = mesh->gmetg + (size_t)e * np * 9;
+        for (int v = 0; v < 4; ++v)
+            gauss_deriv_field(&o, gmg, S + (size_t)(0 * NEQ + v) * ns, S + (size_t)(1 * NEQ + v) * ns,
+                              S + (size_t)(2 * NEQ + v) * ns, G[v][0], G[v][1], G[v][2]);
+        const double *q = JQ + (size_t)e * NEQ * np;
+        const double *jac = mesh->jacob + (size_t)e * np;
+        double sum = 0.0, sum2 = 0.0;
+        for (int l = 0; l < ncg; ++l)
+            for (int m = 0; m < ncg; ++m)
+                for (int n = 0; n < ncg; ++n) {
+                    const int pt = l + ncg * (m + ncg * n);
+                    const double q1 = q[0 * np + pt];
+                    const double udery = ((G[1][1][pt] - (q[1 * np + pt] / q1) * G[0][1][pt])) / q1;
+                    const double uderz = ((G[1][2][pt] - (q[1 * np + pt] / q1) * G[0][2][pt])) / q1;
+                    const double vderx = ((G[2][0][pt] - (q[2 * np + pt] / q1) * G[0][0][pt])) / q1;
+                    const double vderz = ((G[2][2][pt] - (q[2 * np + pt] / q1) * G[0][2][pt])) / q1;
+                    const double wderx = ((G[3][0][pt] - (q[3 * np + pt] / q1) * G[0][0][pt])) / q1;
+                    const double wdery = ((G[3][1][pt] - (q[3 * np + pt] / q1) * G[0][1][pt])) / q1;
+                    const double c1 = (wdery - vderz) / jac[pt];
+                    const double c2 = (uderz - wderx) / jac[pt];
+                    const double c3 = (vderx - udery) / jac[pt];
+                    double vort = c1 * c1 + c2 * c2 + c3 * c3;
+                    vort = vort / jac[pt];
+                    sum2 = sum2 + 1.0 / jac[pt];
+                    sum = sum + vort;
+                }
+        part[2 * e] = sum;
+        part[2 * e + 1] = sum2;
+    }
+    /* the reference's running sum crosses elements in order; summing per
+     * element first changes only the rounding */
+    double sum = 0.0, sum2 = 0.0;
+    for (int e = 0; e < K; ++e) {
+        sum += part[2 * e];
+        sum2 += part[2 * e + 1];
+    }
+    free(part);
+    *eps = 100.0 * (sum / sum2) * (1.0 / ph->re);
     return 0;
 }

@@ -85,8 +85,16 @@ int  or_ev_viscosity(const or_mesh *mesh, const or_phys *ph, const or_ev *ev, co
 int  or_rk_step_ev(const or_mesh *mesh, const or_phys *ph, const or_ev *ev, double *JQ, double *G, double dt,
                    double *stag_lag, int have_lag, double *muh, double *resid, int nthreads);
 int  or_rk_step(const or_mesh *mesh, const or_phys *ph, double *JQ, double *G, double dt,
-                const double *muh, double *resid, int nthreads);
+                const double *muh, double *stag_out, double *resid, int nthreads);
 double or_dtcalc(const or_mesh *mesh, const or_phys *ph, const double *JQ);
+
+/* Diagnostics (caseFiles/tgv/BackstepProblem-stoch.f90:345-361,629-692;
+ * src/Energydissipation/energydissipation.f90:2-89). */
+void or_gauss_legendre(int n, double *t, double *w);
+int  or_ke_weights(int p, double *bmat);                 /* bmat [ncg^3]              */
+int  or_kinetic_energy(const or_mesh *mesh, const double *JQ, double *Ek);
+int  or_enstrophy(const or_mesh *mesh, const or_phys *ph, const double *JQ, const double *stag_lag,
+                  double *eps);
 
 #ifdef __cplusplus
 }

@@ -69,13 +69,18 @@ def lib():
         L.or_riemann.argtypes = [C.POINTER(OrPhys), _dp, C.c_double, _dp, _dp, _dp]
         L.or_vis_flux.argtypes = [C.POINTER(OrPhys), _dp, _dp, _dp, _dp, C.c_double, _dp, _dp, _dp]
         L.or_rhs.argtypes = [C.POINTER(OrMesh), C.POINTER(OrPhys), _dp, _dp, _dp, C.POINTER(OrDebug), C.c_int]
-        L.or_rk_step.argtypes = [C.POINTER(OrMesh), C.POINTER(OrPhys), _dp, _dp, C.c_double, _dp, _dp, C.c_int]
+        L.or_rk_step.argtypes = [C.POINTER(OrMesh), C.POINTER(OrPhys), _dp, _dp, C.c_double, _dp, _dp, _dp,
+                                 C.c_int]
         L.or_dtcalc.argtypes = [C.POINTER(OrMesh), C.POINTER(OrPhys), _dp]
         L.or_ev_viscosity.argtypes = [C.POINTER(OrMesh), C.POINTER(OrPhys), C.POINTER(OrEv), _dp, _dp, _dp, _dp,
                                       _dp, _dp]
         L.or_rk_step_ev.argtypes = [C.POINTER(OrMesh), C.POINTER(OrPhys), C.POINTER(OrEv), _dp, _dp, C.c_double,
                                     _dp, C.c_int, _dp, _dp, C.c_int]
         L.or_dtcalc.restype = C.c_double
+        L.or_gauss_legendre.argtypes = [C.c_int, _dp, _dp]
+        L.or_ke_weights.argtypes = [C.c_int, _dp]
+        L.or_kinetic_energy.argtypes = [C.POINTER(OrMesh), _dp, _dp]
+        L.or_enstrophy.argtypes = [C.POINTER(OrMesh), C.POINTER(OrPhys), _dp, _dp, _dp]
         _lib = L
     return _lib
 
@@ -97,6 +102,9 @@ def ref():
         R.ref_gauss_deriv.argtypes = [C.c_int, _dp, _dp, _dp, _dp, _dp, _dp, _dp]
         R.ref_geometry.argtypes = [C.c_int, _dp, _dp, _dp, _dp, _dp, _dp, _dp]
         R.ref_dtcalc.argtypes = [C.c_int, C.c_double, _dp, _dp, _dp, _dp]
+        R.ref_gaussq.argtypes = [C.c_int, _dp, _dp]
+        R.ref_polyn.argtypes = [C.c_int, C.c_double, C.c_int, _dp]
+        R.ref_polyn.restype = C.c_double
         _ref = R
     return _ref
 
@@ -184,14 +192,20 @@ def rhs(geom: Geometry, JQ: np.ndarray, ph: OrPhys | None = None, muh=None, debu
     return (Qt, out) if debug else Qt
 
 
-def rk_step(geom: Geometry, JQ: np.ndarray, dt: float, ph: OrPhys | None = None, muh=None, nthreads=0):
-    """Advance JQ in place by one RK4(5) step; returns (JQ, G, resid)."""
+def rk_step(geom: Geometry, JQ: np.ndarray, dt: float, ph: OrPhys | None = None, muh=None, nthreads=0,
+            stag_out=None):
+    """Advance JQ in place by one RK4(5) step; returns (JQ, G, resid).
+    stag_out [K,3,5,ns] (optional) receives the last stage's face-averaged
+    staggered states."""
     ph = ph or phys()
     G = np.zeros_like(JQ)
     res = C.c_double(0.0)
     m = geom.cmesh()
     mu = None if muh is None else np.ascontiguousarray(muh, dtype=np.float64)
-    rc = lib().or_rk_step(C.byref(m), C.byref(ph), ptr(JQ), ptr(G), dt, ptr(mu), C.byref(res), nthreads)
+    if stag_out is not None:
+        assert stag_out.flags.c_contiguous and stag_out.dtype == np.float64
+    rc = lib().or_rk_step(C.byref(m), C.byref(ph), ptr(JQ), ptr(G), dt, ptr(mu), ptr(stag_out), C.byref(res),
+                          nthreads)
     assert rc == 0, rc
     return JQ, G, res.value
 
@@ -252,3 +266,33 @@ def rk_step_ev(geom: Geometry, JQ, dt, stag_lag, have_lag, ev: OrEv, ph: OrPhys 
                              int(have_lag), ptr(muh), C.byref(res), nthreads)
     assert rc == 0
     return JQ, muh, res.value
+
+
+# ---------------------------------------------------------------- diagnostics
+def gauss_legendre(n: int):
+    t, w = np.zeros(n), np.zeros(n)
+    lib().or_gauss_legendre(n, ptr(t), ptr(w))
+    return t, w
+
+
+def ke_weights(p: int) -> np.ndarray:
+    """bmatu1b [ncg^3] (index i + ncg(j + ncg l))."""
+    b = np.zeros((p + 1) ** 3)
+    assert lib().or_ke_weights(p, ptr(b)) == 0
+    return b
+
+
+def kinetic_energy(geom: Geometry, JQ) -> float:
+    out = C.c_double(0.0)
+    m = geom.cmesh()
+    assert lib().or_kinetic_energy(C.byref(m), ptr(np.ascontiguousarray(JQ)), C.byref(out)) == 0
+    return out.value
+
+
+def enstrophy(geom: Geometry, JQ, stag_lag, ph: OrPhys | None = None) -> float:
+    ph = ph or phys()
+    out = C.c_double(0.0)
+    m = geom.cmesh()
+    assert lib().or_enstrophy(C.byref(m), C.byref(ph), ptr(np.ascontiguousarray(JQ)),
+                              ptr(np.ascontiguousarray(stag_lag)), C.byref(out)) == 0
+    return out.value

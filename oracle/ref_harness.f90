@@ -176,4 +176,25 @@ contains
       call dtcalc(dt, dd)
    end subroutine ref_dtcalc
 
+   ! gaussq (src/Independentroutines/legendre_routines.f90:98) with the
+   ! arguments of the TGV statistics (BackstepProblem-stoch.f90:327):
+   ! kind = 1 (Legendre), no fixed end points.
+   subroutine ref_gaussq(n, t, w) bind(C, name='ref_gaussq')
+      integer(c_int), value :: n
+      real(c_double) :: t(n), w(n)
+      real(c_double) :: b(n), endpts(2)
+      endpts = 0.0d0
+      call gaussq(1, n, 0.0d0, 0.0d0, 0, endpts, b, t, w)
+   end subroutine ref_gaussq
+
+   ! polyn (src/Independentroutines/baselib.f90:18-48): Lagrange basis k of
+   ! the n points z at x.
+   real(c_double) function ref_polyn(k, x, n, z) bind(C, name='ref_polyn')
+      integer(c_int), value :: k, n
+      real(c_double), value :: x
+      real(c_double) :: z(n)
+      double precision, external :: polyn
+      ref_polyn = polyn(k, x, n, z)
+   end function ref_polyn
+
 end module dsem_ref_harness

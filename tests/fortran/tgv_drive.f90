@@ -1,6 +1,7 @@
 ! Fortran driver of the GPU library through the iso_c_binding module: reads
 ! a mesh and a J*Q state in the reference's padded layout, takes nsteps
-! steps with the tstep-compatible wrapper and writes the compact state.
+! steps with the tstep-compatible wrapper, takes the TGV statistics after
+! every step (turbk, enstrophy dissipation) and writes the compact state.
 ! Input/output are stream files written/read by tests/test_gpu_fortran.py.
 program tgv_drive
    use iso_c_binding
@@ -9,7 +10,7 @@ program tgv_drive
    integer(c_int32_t) :: hdr(4)
    integer(c_int32_t), allocatable, target :: me(:), mf(:), ms(:), mn(:)
    real(c_double), allocatable, target :: corners(:), jq_pad(:), jq_out(:)
-   real(c_double) :: phys(4), dt, resid, t
+   real(c_double) :: phys(4), dt, resid, t, turbk, eps
    real(c_double), allocatable :: dtv(:)
    integer(c_int) :: rc, ierr, nsteps, k
    type(dsem_mesh_desc) :: desc
@@ -42,6 +43,8 @@ program tgv_drive
    end if
    rc = dsem_set_state(ctx, jq_pad, 1)
    if (rc /= DSEM_OK) stop 2
+   rc = dsem_keep_lagged_states(ctx, 1)
+   if (rc /= DSEM_OK) stop 5
    allocate(dtv(hdr(1)))
    dtv = dt
    t = 0.0d0
@@ -51,13 +54,18 @@ program tgv_drive
          print *, 'tstep failed: ', trim(dsem_last_error(ctx))
          stop 3
       end if
+      call dsem_energy(ctx, turbk, eps, ierr)
+      if (ierr /= DSEM_OK) then
+         print *, 'energy failed: ', trim(dsem_last_error(ctx))
+         stop 6
+      end if
       t = t + dt
    end do
    rc = dsem_get_state(ctx, jq_out, 0)
    if (rc /= DSEM_OK) stop 4
    rc = dsem_destroy(ctx)
    open(11, file=trim(fout), access='stream', form='unformatted', status='replace')
-   write(11) resid, jq_out
+   write(11) resid, turbk, eps, jq_out
    close(11)
-   print '(a,i0,a,es14.6)', 'fortran driver: ', nsteps, ' steps, resid ', resid
+   print '(a,i0,a,es14.6,a,es22.15)', 'fortran driver: ', nsteps, ' steps, resid ', resid, ' turbk ', turbk
 end program tgv_drive

@@ -43,13 +43,16 @@ def test_fortran_driver_matches_python_binding_and_oracle(tmp_path):
     print(out.stdout, out.stderr, flush=True)
     assert out.returncode == 0
     raw = np.fromfile(fout, dtype=np.float64)
-    resid_f, jq_f = raw[0], raw[1:].reshape(mesh.K, 5, -1)
+    resid_f, ek_f, eps_f, jq_f = raw[0], raw[1], raw[2], raw[3:].reshape(mesh.K, 5, -1)
     with DSEM(mesh, re=400.0, metrics="per_point") as s:
         s.set_state(JQ)
+        s.keep_lagged_states(True)
         for _ in range(nsteps):
             r = s.tstep(0.0, dt)
         jq_p = s.get_state()
+        ek_p, eps_p = s.diagnostics()
     assert np.array_equal(jq_f, jq_p) and resid_f == r
+    assert ek_f == ek_p and eps_f == eps_p
     ref = JQ.copy()
     for _ in range(nsteps):
         ref, _, _ = O.rk_step(g, ref, dt, ph)
