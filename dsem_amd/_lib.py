@@ -48,12 +48,29 @@ class DsemError(RuntimeError):
 _lib = None
 
 
+def _share_torch_hip_runtime() -> None:
+    """One HIP runtime per process.  PyTorch-ROCm ships its own
+    libamdhip64.so (soname libamdhip64.so.7, the same as the system's) and
+    asks for it by the unversioned name, so if this library pulled the
+    system copy in first, torch would load a second runtime that sees no GPU.
+    Loading torch's copy first (global symbols) makes both bind to it; torch
+    itself is not imported."""
+    import importlib.util
+    spec = importlib.util.find_spec("torch")
+    if spec is None or not spec.submodule_search_locations:
+        return
+    rt = os.path.join(list(spec.submodule_search_locations)[0], "lib", "libamdhip64.so")
+    if os.path.exists(rt):
+        C.CDLL(rt, mode=C.RTLD_GLOBAL)
+
+
 def load() -> C.CDLL:
     global _lib
     if _lib is not None:
         return _lib
     if not os.path.exists(LIB_PATH):
         raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m dsem_amd.build`")
+    _share_torch_hip_runtime()
     L = C.CDLL(LIB_PATH)
     vp = C.c_void_p
     sig = {
@@ -71,6 +88,13 @@ def load() -> C.CDLL:
         "dsem_get_lagged_states": ([vp, _dp], C.c_int),
         "dsem_reduce_ke": ([vp, _dp, _dp], C.c_int),
         "dsem_keep_lagged_states": ([vp, C.c_int], C.c_int),
+        "dsem_rhs_vjp": ([vp, _dp, _dp, _dp], C.c_int),
+        "dsem_rhs_vjp_device": ([vp, vp, vp, vp, _dp], C.c_int),
+        "dsem_rk_step_vjp": ([vp, C.c_double, _dp, _dp], C.c_int),
+        "dsem_rk_step_vjp_device": ([vp, C.c_double, vp, _dp], C.c_int),
+        "dsem_set_state_device": ([vp, vp], C.c_int),
+        "dsem_get_state_device": ([vp, vp], C.c_int),
+        "dsem_quadrature_weights": ([vp, _dp], C.c_int),
         "dsem_rhs": ([vp, C.c_double, _dp], C.c_int),
         "dsem_rk_step": ([vp, C.c_double, C.c_double, _dp], C.c_int),
         "dsem_rk_steps": ([vp, C.c_double, C.c_double, C.c_int, _dp], C.c_int),

@@ -57,6 +57,71 @@ __constant__ double c_D[9][90];   // D[k*nc + j]
 __constant__ double c_B[9][90];   // B[j*ncg + k]
 __constant__ double c_M[9][81];   // M[k*ncg + k2]
 __constant__ double c_cx[9][10];  // Lobatto points
+// single-precision copies for the mixed contraction modes
+__constant__ float c_Df[9][90];
+__constant__ float c_Bf[9][90];
+__constant__ float c_Mf[9][81];
+__constant__ double c_Dint[9][9];  // sum over the interior Lobatto j of D[k][j]
+
+// Contraction arithmetic of the volume kernel, by precision mode:
+//   PM 0  FP64 (the drop-in path);
+//   PM 1  FP32 multiply-add, FP32 accumulation;
+//   PM 2  BF16 split products: a*b ~ ah*bh + ah*bl + al*bh with
+//         x = xh + xl, xh = bf16(x), xl = bf16(x - xh); each product of two
+//         bf16 values is exact in FP32, accumulation in FP32.
+// In the mixed modes the contracted fields are shifted by an element
+// constant first (see k_volume), so that FP32 rounding acts on the
+// variation within the element and not on the low-Mach background.
+template <int PM>
+struct Ctr {
+    using T = float;
+    using A = float;
+};
+template <>
+struct Ctr<0> {
+    using T = double;
+    using A = double;
+};
+
+__device__ __forceinline__ float bf16_round(float x)
+{
+    unsigned u = __float_as_uint(x);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return __uint_as_float(u & 0xffff0000u);
+}
+
+template <int PM>
+__device__ __forceinline__ typename Ctr<PM>::A mac(typename Ctr<PM>::T a, typename Ctr<PM>::T b,
+                                                    typename Ctr<PM>::A acc)
+{
+    if constexpr (PM == 0) {
+        return fma(a, b, acc);
+    } else if constexpr (PM == 1) {
+        return fmaf(a, b, acc);
+    } else {
+        const float ah = bf16_round(a), al = bf16_round(a - ah);
+        const float bh = bf16_round(b), bl = bf16_round(b - bh);
+        acc = fmaf(ah, bh, acc);
+        acc = fmaf(ah, bl, acc);
+        return fmaf(al, bh, acc);
+    }
+}
+
+template <int PM>
+__device__ __forceinline__ typename Ctr<PM>::T opB(int p, int idx)
+{
+    if constexpr (PM == 0) return c_B[p][idx]; else return c_Bf[p][idx];
+}
+template <int PM>
+__device__ __forceinline__ typename Ctr<PM>::T opD(int p, int idx)
+{
+    if constexpr (PM == 0) return c_D[p][idx]; else return c_Df[p][idx];
+}
+template <int PM>
+__device__ __forceinline__ typename Ctr<PM>::T opM(int p, int idx)
+{
+    if constexpr (PM == 0) return c_M[p][idx]; else return c_Mf[p][idx];
+}
 
 template <int P>
 struct Sz {
@@ -274,14 +339,17 @@ __device__ __forceinline__ void load_physical_state(const double *__restrict__ J
 // One staggered grid D of the fused volume kernel: fluxes at the nc*ncg^2
 // points of grid D into LDS (interior) or the viscous face traces (faces),
 // then the interior part of the divergence along D added into acc.
-template <int P, int D>
+template <int P, int D, int PM>
 __device__ __forceinline__ void volume_grid(const DevMesh &m, const DevPhys &ph, const double *__restrict__ QA,
                                             const double *__restrict__ muh, double *__restrict__ FV,
-                                            double *__restrict__ Slag,
-                                            const double *sQ, const double *sG, double *sF, int e,
-                                            const int *mort, double (&acc)[(Sz<P>::NP + VBLOCK - 1) / VBLOCK][5])
+                                            double *__restrict__ Slag, const typename Ctr<PM>::T *sQ,
+                                            const typename Ctr<PM>::T *sG, typename Ctr<PM>::T *sF,
+                                            const double *sBar, int e, const int *mort,
+                                            double (&acc)[(Sz<P>::NP + VBLOCK - 1) / VBLOCK][5])
 {
     using S = Sz<P>;
+    using CT = typename Ctr<PM>::T;
+    using CA = typename Ctr<PM>::A;
     constexpr int R = (S::NP + VBLOCK - 1) / VBLOCK;
     const int tid = threadIdx.x;
     const int nsm = m.ms ? S::NS : 1, nfm = m.ms ? S::NF : 1;
@@ -295,9 +363,9 @@ __device__ __forceinline__ void volume_grid(const DevMesh &m, const DevPhys &ph,
         else { a = s % S::NCG; b = (s / S::NCG) % S::NCG; j = s / (S::NCG * S::NCG); }
         const bool face = (j == 0 || j == S::NC - 1);
         const int q = a + S::NCG * b;
-        double Bj[S::NCG];
+        CT Bj[S::NCG];
 #pragma unroll
-        for (int k = 0; k < S::NCG; ++k) Bj[k] = c_B[P][j * S::NCG + k];
+        for (int k = 0; k < S::NCG; ++k) Bj[k] = opB<PM>(P, j * S::NCG + k);
         const int fc = face_of<P>(D, j);
         const int mo = mort[fc - 1];
         double Qv[5];
@@ -307,10 +375,10 @@ __device__ __forceinline__ void volume_grid(const DevMesh &m, const DevPhys &ph,
         } else {
 #pragma unroll
             for (int v = 0; v < 5; ++v) {
-                double t = 0.0;
+                CA t = 0;
 #pragma unroll
-                for (int k = 0; k < S::NCG; ++k) t = fma(Bj[k], sQ[v * S::NP + gidx<P>(D, k, a, b)], t);
-                Qv[v] = t;
+                for (int k = 0; k < S::NCG; ++k) t = mac<PM>(Bj[k], sQ[v * S::NP + gidx<P>(D, k, a, b)], t);
+                if constexpr (PM == 0) Qv[v] = t; else Qv[v] = sBar[v] + (double)t;
                 if (v == 1 || v == 3) __builtin_amdgcn_sched_barrier(0);
             }
         }
@@ -330,11 +398,11 @@ __device__ __forceinline__ void volume_grid(const DevMesh &m, const DevPhys &ph,
             double Gc[5];
 #pragma unroll
             for (int v = 0; v < 5; ++v) {
-                const double *gl = sG + (c * 5 + v) * S::NP;
-                double t = 0.0;
+                const CT *gl = sG + (c * 5 + v) * S::NP;
+                CA t = 0;
 #pragma unroll
-                for (int k = 0; k < S::NCG; ++k) t = fma(Bj[k], gl[gidx<P>(D, k, a, b)], t);
-                Gc[v] = t;
+                for (int k = 0; k < S::NCG; ++k) t = mac<PM>(Bj[k], gl[gidx<P>(D, k, a, b)], t);
+                Gc[v] = (double)t;
             }
             double o[4];
             prim_gradient(ph, pr, Gc, o);
@@ -378,7 +446,10 @@ __device__ __forceinline__ void volume_grid(const DevMesh &m, const DevPhys &ph,
             double fi[5];
             euler_from_primitive(pr, Qv, g0, g1, g2, fi);
 #pragma unroll
-            for (int v = 0; v < 5; ++v) sF[v * S::NS + s] = fi[v] - rv[v] * ph.inv_re;
+            for (int v = 0; v < 5; ++v) {
+                if constexpr (PM == 0) sF[v * S::NS + s] = fi[v] - rv[v] * ph.inv_re;
+                else sF[v * S::NS + s] = (CT)((fi[v] - rv[v] * ph.inv_re) - sBar[5 + D * 5 + v]);
+            }
         }
     }
     __syncthreads();
@@ -390,15 +461,16 @@ __device__ __forceinline__ void volume_grid(const DevMesh &m, const DevPhys &ph,
             const int k = D == 0 ? n : (D == 1 ? mi : l);
             const int a = D == 0 ? mi : n;
             const int b = D == 2 ? mi : l;
-            double Dk[S::NC];
+            CT Dk[S::NC];
 #pragma unroll
-            for (int j = 1; j < S::NC - 1; ++j) Dk[j] = c_D[P][k * S::NC + j];
+            for (int j = 1; j < S::NC - 1; ++j) Dk[j] = opD<PM>(P, k * S::NC + j);
 #pragma unroll
             for (int v = 0; v < 5; ++v) {
-                double t = 0.0;
+                CA t = 0;
 #pragma unroll
-                for (int j = 1; j < S::NC - 1; ++j) t = fma(Dk[j], sF[v * S::NS + sidx<P>(D, j, a, b)], t);
-                acc[r][v] -= t;
+                for (int j = 1; j < S::NC - 1; ++j) t = mac<PM>(Dk[j], sF[v * S::NS + sidx<P>(D, j, a, b)], t);
+                if constexpr (PM == 0) acc[r][v] -= t;
+                else acc[r][v] -= (double)t + sBar[5 + D * 5 + v] * c_Dint[P][k];
             }
         }
     }
@@ -408,23 +480,57 @@ __device__ __forceinline__ void volume_grid(const DevMesh &m, const DevPhys &ph,
 // SolnDerivsc.f90:14-83), then for each staggered grid the interpolated
 // state, prolonged gradients, contravariant Euler + viscous fluxes, the
 // viscous face traces r/norm and the interior part of the divergence.
-template <int P>
+//
+// Mixed modes (PM 1, 2): the LDS tiles hold FP32 values and the
+// contractions run in FP32 on element-shifted fields, with the shifts added
+// back in FP64 -- exact in exact arithmetic because B reproduces constants
+// and the rows of D annihilate them:
+//   Q     - Qbar   (Qbar = Q at the element's central Gauss point);
+//   faces QA - Qbar in the gradient (D Qbar = 0 is dropped);
+//   flux  F - Fbar_d (Fbar_d = contravariant Euler flux of Qbar with the
+//         metric row at staggered point 0 of grid d); the divergence adds
+//         Fbar_d * sum_{interior j} D[k][j] back in FP64.
+// Pointwise physics, the gradient metric combination inputs, the RK state
+// and all face kernels stay FP64.
+template <int P, int PM>
 __global__ __launch_bounds__(VBLOCK) void k_volume(const double *__restrict__ JQ, DevMesh m, DevPhys ph,
                                                     const double *__restrict__ QA, const double *__restrict__ muh,
                                                     double *__restrict__ FV, double *__restrict__ Qtp,
                                                     double *__restrict__ Slag)
 {
     using S = Sz<P>;
+    using CT = typename Ctr<PM>::T;
+    using CA = typename Ctr<PM>::A;
     constexpr int R = (S::NP + VBLOCK - 1) / VBLOCK;
-    __shared__ double sQ[NEQ * S::NP];        // physical Q at Gauss points
-    __shared__ double sG[15 * S::NP];         // dQ_v/dx_c at Gauss points, (c*5+v)
-    __shared__ double sF[NEQ * S::NS];        // interior total flux of the current grid
+    __shared__ CT sQ[NEQ * S::NP];            // physical Q at Gauss points (minus Qbar when mixed)
+    __shared__ CT sG[15 * S::NP];             // dQ_v/dx_c at Gauss points, (c*5+v)
+    __shared__ CT sF[NEQ * S::NS];            // interior total flux of the current grid (minus Fbar)
+    __shared__ double sBar[5 + 15];           // mixed modes: Qbar[5], Fbar[3][5]
     const int e = blockIdx.x;
     const int tid = threadIdx.x;
+    const double *jac = m.jacob + (size_t)e * (m.ms ? S::NP : 1);
+    if constexpr (PM != 0) {
+        constexpr int pc = (S::NCG / 2) * (1 + S::NCG + S::NCG * S::NCG);
+        if (tid < 5) sBar[tid] = JQ[((size_t)e * NEQ + tid) * S::NP + pc] * jac[pc * m.ms];
+        __syncthreads();
+        if (tid < 3) {
+            const int nsm = m.ms ? S::NS : 1;
+            const double *gmd = (tid == 0 ? m.gmet0 : (tid == 1 ? m.gmet1 : m.gmet2)) + (size_t)e * 3 * nsm;
+            double qb[5], fb[5];
+#pragma unroll
+            for (int v = 0; v < 5; ++v) qb[v] = sBar[v];
+            const Prim pb = primitives(ph, qb);
+            euler_from_primitive(pb, qb, gmd[0], gmd[nsm], gmd[2 * nsm], fb);
+#pragma unroll
+            for (int v = 0; v < 5; ++v) sBar[5 + tid * 5 + v] = fb[v];
+        }
+    }
     {
         const double *q = JQ + (size_t)e * NEQ * S::NP;
-        const double *jac = m.jacob + (size_t)e * (m.ms ? S::NP : 1);
-        for (int i = tid; i < NEQ * S::NP; i += VBLOCK) sQ[i] = q[i] * jac[(i % S::NP) * m.ms];
+        for (int i = tid; i < NEQ * S::NP; i += VBLOCK) {
+            const double qp = q[i] * jac[(i % S::NP) * m.ms];
+            if constexpr (PM == 0) sQ[i] = qp; else sQ[i] = (CT)(qp - sBar[i / S::NP]);
+        }
     }
     const int *e2m = m.e2m + (size_t)e * 6;
     int mort[6];
@@ -440,29 +546,34 @@ __global__ __launch_bounds__(VBLOCK) void k_volume(const double *__restrict__ JQ
         for (int pt = tid; pt < S::NP; pt += VBLOCK) {
             const int n = pt % S::NCG, mi = (pt / S::NCG) % S::NCG, l = pt / (S::NCG * S::NCG);
             const int pq = pt * m.ms;
-            double out[3][5];
+            CA out[3][5];
 #pragma unroll
             for (int d = 0; d < 3; ++d) {
                 const int k = d == 0 ? n : (d == 1 ? mi : l);
                 const int a = d == 0 ? mi : n;
                 const int b = d == 2 ? mi : l;
                 const int q = a + S::NCG * b;
-                double Mk[S::NCG];
+                CT Mk[S::NCG];
 #pragma unroll
-                for (int k2 = 0; k2 < S::NCG; ++k2) Mk[k2] = c_M[P][k * S::NCG + k2];
-                const double d0 = c_D[P][k * S::NC], d1 = c_D[P][k * S::NC + S::NC - 1];
+                for (int k2 = 0; k2 < S::NCG; ++k2) Mk[k2] = opM<PM>(P, k * S::NCG + k2);
+                const CT d0 = opD<PM>(P, k * S::NC), d1 = opD<PM>(P, k * S::NC + S::NC - 1);
                 const double *qa0 = QA + (size_t)mort[face_of<P>(d, 0) - 1] * NEQ * S::NF + q;
                 const double *qa1 = QA + (size_t)mort[face_of<P>(d, 1) - 1] * NEQ * S::NF + q;
-                double gd[3];
+                CA gd[3];
 #pragma unroll
-                for (int c = 0; c < 3; ++c) gd[c] = gg[(d * 3 + c) * npm + pq];
+                for (int c = 0; c < 3; ++c) gd[c] = (CA)gg[(d * 3 + c) * npm + pq];
 #pragma unroll
                 for (int v = 0; v < 5; ++v) {
-                    double t = 0.0;
+                    CA t = 0;
 #pragma unroll
-                    for (int k2 = 0; k2 < S::NCG; ++k2) t = fma(Mk[k2], sQ[v * S::NP + gidx<P>(d, k2, a, b)], t);
-                    t = fma(d0, qa0[v * S::NF], t);
-                    t = fma(d1, qa1[v * S::NF], t);
+                    for (int k2 = 0; k2 < S::NCG; ++k2) t = mac<PM>(Mk[k2], sQ[v * S::NP + gidx<P>(d, k2, a, b)], t);
+                    if constexpr (PM == 0) {
+                        t = fma(d0, qa0[v * S::NF], t);
+                        t = fma(d1, qa1[v * S::NF], t);
+                    } else {
+                        t = mac<PM>(d0, (CT)(qa0[v * S::NF] - sBar[v]), t);
+                        t = mac<PM>(d1, (CT)(qa1[v * S::NF] - sBar[v]), t);
+                    }
 #pragma unroll
                     for (int c = 0; c < 3; ++c) out[c][v] = d == 0 ? gd[c] * t : fma(gd[c], t, out[c][v]);
                 }
@@ -471,7 +582,7 @@ __global__ __launch_bounds__(VBLOCK) void k_volume(const double *__restrict__ JQ
 #pragma unroll
             for (int c = 0; c < 3; ++c)
 #pragma unroll
-                for (int v = 0; v < 5; ++v) sG[(c * 5 + v) * S::NP + pt] = out[c][v];
+                for (int v = 0; v < 5; ++v) sG[(c * 5 + v) * S::NP + pt] = (CT)out[c][v];
         }
     }
 
@@ -480,9 +591,9 @@ __global__ __launch_bounds__(VBLOCK) void k_volume(const double *__restrict__ JQ
     for (int r = 0; r < R; ++r)
 #pragma unroll
         for (int v = 0; v < 5; ++v) acc[r][v] = 0.0;
-    volume_grid<P, 0>(m, ph, QA, muh, FV, Slag, sQ, sG, sF, e, mort, acc);
-    volume_grid<P, 1>(m, ph, QA, muh, FV, Slag, sQ, sG, sF, e, mort, acc);
-    volume_grid<P, 2>(m, ph, QA, muh, FV, Slag, sQ, sG, sF, e, mort, acc);
+    volume_grid<P, 0, PM>(m, ph, QA, muh, FV, Slag, sQ, sG, sF, sBar, e, mort, acc);
+    volume_grid<P, 1, PM>(m, ph, QA, muh, FV, Slag, sQ, sG, sF, sBar, e, mort, acc);
+    volume_grid<P, 2, PM>(m, ph, QA, muh, FV, Slag, sQ, sG, sF, sBar, e, mort, acc);
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         const int pt = tid + r * VBLOCK;
@@ -527,60 +638,96 @@ struct StageArgs {
 // g = a g + Qt, JQ += c dt g (updatea.f90:213-227).  With TRACES the updated
 // state is staged in LDS and the face traces of the next stage are computed
 // here, so the state is not read again by k_traces.
+// HBM-bound: 512 work-items per element and UBLK independent points per
+// work-item in flight (all loads of a batch issued before any store), so that
+// enough bytes are outstanding per CU to approach the HBM bandwidth.
+constexpr int UBLOCK = 512;
+constexpr int UBLK = 4;
+
 template <int P, bool TRACES>
-__global__ __launch_bounds__(BLOCK) void k_update(const double *__restrict__ Qtp, const double *__restrict__ FT, DevMesh m,
-                                                   StageArgs st, double *__restrict__ JQ, double *__restrict__ G,
-                                                   double *__restrict__ Qt, double *__restrict__ T)
+__global__ __launch_bounds__(UBLOCK) void k_update(const double *__restrict__ Qtp, const double *__restrict__ FT, DevMesh m,
+                                                    StageArgs st, double *__restrict__ JQ, double *__restrict__ G,
+                                                    double *__restrict__ Qt, double *__restrict__ T)
 {
     using S = Sz<P>;
-    __shared__ double sFT[6 * NEQ * S::NF];
-    __shared__ double sQ[TRACES ? NEQ * S::NP : 1];
-    const int e = blockIdx.x;
+    constexpr int NQ = NEQ * S::NP, NFT = 6 * NEQ * S::NF;
+    __shared__ double sFT[NFT];
+    __shared__ double sQ[TRACES ? NQ : 1];
+    const int e = blockIdx.x, tid = threadIdx.x;
     const int *e2m = m.e2m + (size_t)e * 6;
-    for (int i = threadIdx.x; i < 6 * NEQ * S::NF; i += BLOCK) {
-        const int f = i / (NEQ * S::NF), r = i % (NEQ * S::NF);
-        const int c = e2m[f];
-        double val = FT[(size_t)(c >> 1) * NEQ * S::NF + r];
-        if ((c & 1) && m.m_sign[c >> 1] == -1) val = -val;
-        sFT[i] = val;
+    for (int base = tid; base < NFT; base += UBLOCK * UBLK) {
+        double val[UBLK];
+#pragma unroll
+        for (int u = 0; u < UBLK; ++u) {
+            const int i = base + u * UBLOCK;
+            if (i < NFT) {
+                const int f = i / (NEQ * S::NF), r = i % (NEQ * S::NF);
+                const int c = e2m[f];
+                val[u] = FT[(size_t)(c >> 1) * NEQ * S::NF + r];
+                if ((c & 1) && m.m_sign[c >> 1] == -1) val[u] = -val[u];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < UBLK; ++u) {
+            const int i = base + u * UBLOCK;
+            if (i < NFT) sFT[i] = val[u];
+        }
     }
     __syncthreads();
     const double *jac = m.jacob + (size_t)e * (m.ms ? S::NP : 1);
+    const size_t e0 = (size_t)e * NQ;
     double mx = 0.0;
     bool bad = false;
-    for (int i = threadIdx.x; i < NEQ * S::NP; i += BLOCK) {
-        const int v = i / S::NP, pt = i % S::NP;
-        const int n = pt % S::NCG, mi = (pt / S::NCG) % S::NCG, l = pt / (S::NCG * S::NCG);
-        double qt = Qtp[(size_t)e * NEQ * S::NP + i];
+    for (int base = tid; base < NQ; base += UBLOCK * UBLK) {
+        double qv[UBLK], gv[UBLK], jv[UBLK], jc[UBLK];
 #pragma unroll
-        for (int d = 0; d < 3; ++d) {
-            const int k = d == 0 ? n : (d == 1 ? mi : l);
-            const int a = d == 0 ? mi : n;
-            const int b = d == 2 ? mi : l;
-            const int q = a + S::NCG * b;
-            const double F0 = sFT[((face_of<P>(d, 0) - 1) * NEQ + v) * S::NF + q];
-            const double F1 = sFT[((face_of<P>(d, 1) - 1) * NEQ + v) * S::NF + q];
-            qt = qt - fma(c_D[P][k * S::NC + 0], F0, c_D[P][k * S::NC + S::NC - 1] * F1);
+        for (int u = 0; u < UBLK; ++u) {
+            const int i = base + u * UBLOCK;
+            if (i < NQ) {
+                qv[u] = Qtp[e0 + i];
+                if (st.rk) {
+                    gv[u] = st.first ? 0.0 : G[e0 + i];
+                    jv[u] = JQ[e0 + i];
+                    if (TRACES) jc[u] = jac[(i % S::NP) * m.ms];
+                }
+            }
         }
-        const size_t gi = (size_t)e * NEQ * S::NP + i;
-        if (st.rk) {
-            const double g = st.first ? qt : fma(st.a, G[gi], qt);
-            G[gi] = g;
-            const double jq = fma(st.cdt, g, JQ[gi]);
-            JQ[gi] = jq;
-            if (TRACES) sQ[i] = jq * jac[pt * m.ms];
-        } else {
-            Qt[gi] = qt;
+#pragma unroll
+        for (int u = 0; u < UBLK; ++u) {
+            const int i = base + u * UBLOCK;
+            if (i >= NQ) continue;
+            const int v = i / S::NP, pt = i % S::NP;
+            const int n = pt % S::NCG, mi = (pt / S::NCG) % S::NCG, l = pt / (S::NCG * S::NCG);
+            double qt = qv[u];
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                const int k = d == 0 ? n : (d == 1 ? mi : l);
+                const int a = d == 0 ? mi : n;
+                const int b = d == 2 ? mi : l;
+                const int q = a + S::NCG * b;
+                const double F0 = sFT[((face_of<P>(d, 0) - 1) * NEQ + v) * S::NF + q];
+                const double F1 = sFT[((face_of<P>(d, 1) - 1) * NEQ + v) * S::NF + q];
+                qt = qt - fma(c_D[P][k * S::NC + 0], F0, c_D[P][k * S::NC + S::NC - 1] * F1);
+            }
+            if (st.rk) {
+                const double g = st.first ? qt : fma(st.a, gv[u], qt);
+                G[e0 + i] = g;
+                const double jq = fma(st.cdt, g, jv[u]);
+                JQ[e0 + i] = jq;
+                if (TRACES) sQ[i] = jq * jc[u];
+            } else {
+                Qt[e0 + i] = qt;
+            }
+            mx = fmax(mx, fabs(qt));
+            bad = bad || !isfinite(qt);
         }
-        mx = fmax(mx, fabs(qt));
-        bad = bad || !isfinite(qt);
     }
     if (bad && st.nonfinite) atomicOr(st.nonfinite, 1u);
     if (st.resid && e == m.K - 1) atomicMax(st.resid, (unsigned long long)__double_as_longlong(mx));
     if (TRACES) {
         __syncthreads();
-        double *t = T + (size_t)e * 6 * NEQ * S::NF;
-        for (int i = threadIdx.x; i < 6 * NEQ * S::NF; i += BLOCK) {
+        double *t = T + (size_t)e * NFT;
+        for (int i = tid; i < NFT; i += UBLOCK) {
             const int qf = i % S::NF, v = (i / S::NF) % NEQ, f = i / (NEQ * S::NF) + 1;
             const int a = qf % S::NCG, b = qf / S::NCG;
             const int d = (f == 4 || f == 6) ? 0 : ((f == 1 || f == 2) ? 1 : 2);
@@ -932,14 +1079,16 @@ __global__ __launch_bounds__(BLOCK) void k_dtcalc(const double *__restrict__ JQ,
     atomicMin(out, (unsigned long long)__double_as_longlong(best));
 }
 
+#include "adjoint.inc"
+
 }  // namespace
 
 // ====================================================================== host
 namespace {
 
-enum KernelId { KID_TRACES = 0, KID_FACE_INV, KID_VOLUME, KID_FACE_VISC, KID_UPDATE, KID_DTCALC, KID_EV, KID_DIAG, KID_COUNT };
+enum KernelId { KID_TRACES = 0, KID_FACE_INV, KID_VOLUME, KID_FACE_VISC, KID_UPDATE, KID_DTCALC, KID_EV, KID_DIAG, KID_ADJ, KID_COUNT };
 const char *const kKernelNames[KID_COUNT] = {"k_traces", "k_face_inv", "k_volume", "k_face_visc",
-                                             "k_update", "k_dtcalc", "k_ev", "k_diag"};
+                                             "k_update", "k_dtcalc", "k_ev", "k_diag", "k_adj"};
 
 template <typename T>
 struct DevBuf {
@@ -967,6 +1116,10 @@ struct dsem_ctx {
     bool have_lag = false;
     bool keep_lag = false;                     // save the last stage's staggered states without EV
     double *bmat = nullptr, *diag_part = nullptr, *diag_out = nullptr;   // diagnostics
+    // adjoint buffers (allocated by the first VJP call)
+    double *aW = nullptr, *aQVb = nullptr, *aGVb = nullptr, *aFTbe = nullptr, *aFVb = nullptr, *aQAbe = nullptr;
+    double *aTb = nullptr, *aFIb = nullptr, *aJQvb = nullptr, *aNuPart = nullptr, *aNu = nullptr;
+    double *aStages = nullptr, *aGb = nullptr, *aLam = nullptr, *aOut = nullptr;
     int *e2m = nullptr, *m_elem = nullptr, *m_face = nullptr, *m_sign = nullptr, *m_nsign = nullptr;
     unsigned long long *d_scalars = nullptr;   // [0] resid bits, [1] dt bits, [2] nonfinite flag
     std::vector<void *> allocations;
@@ -1081,12 +1234,19 @@ int run_rhs(dsem_ctx *c, double *JQ, const StageArgs &st, double *Qt_out, bool t
     if (!traces_ready) TRY(launch(c, KID_TRACES, [&] { k_traces<P><<<K, BLOCK, 0, s>>>(JQ, m, c->T); }));
     TRY(launch(c, KID_FACE_INV, [&] { k_face_inv<P><<<face_blocks, BLOCK, 0, s>>>(c->T, m, ph, c->FI, c->QA); }));
     double *slag = save_lag ? c->Slag : nullptr;
-    TRY(launch(c, KID_VOLUME, [&] { k_volume<P><<<K, VBLOCK, 0, s>>>(JQ, m, ph, c->QA, muh, c->FV, c->Qtp, slag); }));
+    TRY(launch(c, KID_VOLUME, [&] {
+        if (c->params.precision == DSEM_PREC_FP32)
+            k_volume<P, 1><<<K, VBLOCK, 0, s>>>(JQ, m, ph, c->QA, muh, c->FV, c->Qtp, slag);
+        else if (c->params.precision == DSEM_PREC_BF16X3)
+            k_volume<P, 2><<<K, VBLOCK, 0, s>>>(JQ, m, ph, c->QA, muh, c->FV, c->Qtp, slag);
+        else
+            k_volume<P, 0><<<K, VBLOCK, 0, s>>>(JQ, m, ph, c->QA, muh, c->FV, c->Qtp, slag);
+    }));
     TRY(launch(c, KID_FACE_VISC, [&] { k_face_visc<P><<<face_blocks, BLOCK, 0, s>>>(c->FV, c->FI, m, ph, c->FT); }));
     if (make_traces)
-        TRY(launch(c, KID_UPDATE, [&] { k_update<P, true><<<K, BLOCK, 0, s>>>(c->Qtp, c->FT, m, st, JQ, c->G, Qt_out, c->T); }));
+        TRY(launch(c, KID_UPDATE, [&] { k_update<P, true><<<K, UBLOCK, 0, s>>>(c->Qtp, c->FT, m, st, JQ, c->G, Qt_out, c->T); }));
     else
-        TRY(launch(c, KID_UPDATE, [&] { k_update<P, false><<<K, BLOCK, 0, s>>>(c->Qtp, c->FT, m, st, JQ, c->G, Qt_out, c->T); }));
+        TRY(launch(c, KID_UPDATE, [&] { k_update<P, false><<<K, UBLOCK, 0, s>>>(c->Qtp, c->FT, m, st, JQ, c->G, Qt_out, c->T); }));
     return DSEM_OK;
 }
 
@@ -1174,6 +1334,63 @@ int diag_dispatch(dsem_ctx *c, bool vorticity)
     DISPATCH_P(c, run_diag, vorticity);
 }
 
+int adjoint_alloc(dsem_ctx *c)
+{
+    if (c->aW) return DSEM_OK;
+    const size_t K = c->K, n = K * NEQ * c->np, nfe = K * 6 * NEQ * c->nf;
+    const size_t face_blocks = ((size_t)c->M * c->nf + BLOCK - 1) / BLOCK;
+    TRY(dev_alloc(c, &c->aW, K * 15 * c->np));
+    TRY(dev_alloc(c, &c->aQVb, K * 3 * NEQ * c->ns));
+    TRY(dev_alloc(c, &c->aGVb, K * 3 * 15 * c->ns));
+    TRY(dev_alloc(c, &c->aFTbe, nfe));
+    TRY(dev_alloc(c, &c->aFVb, nfe));
+    TRY(dev_alloc(c, &c->aQAbe, nfe));
+    TRY(dev_alloc(c, &c->aTb, nfe));
+    TRY(dev_alloc(c, &c->aFIb, (size_t)c->M * NEQ * c->nf));
+    TRY(dev_alloc(c, &c->aJQvb, n));
+    TRY(dev_alloc(c, &c->aNuPart, K + face_blocks));
+    TRY(dev_alloc(c, &c->aNu, 16));
+    TRY(dev_alloc(c, &c->aStages, 5 * n));
+    TRY(dev_alloc(c, &c->aGb, n));
+    TRY(dev_alloc(c, &c->aLam, n));
+    TRY(dev_alloc(c, &c->aOut, n));
+    return DSEM_OK;
+}
+
+// Adjoint of one RHS evaluation at state JQ (device) for the cotangent in
+// Gb.  Plain VJP (Qb == nullptr): Out = (dR/dJQ)^T Gb.  RK stage reverse
+// (Qb != nullptr): Gb <- Gb + cdt Qb, Qb += (dR/dJQ)^T Gb, Gb <- a Gb.
+// nu_slot receives <Gb, dR/dnu> (device, one double).
+template <int P>
+int run_rhs_vjp(dsem_ctx *c, double *JQ, double *Gb, double *Qb, double cdt, double a, double *Out, double *nu_slot)
+{
+    using S = Sz<P>;
+    const DevMesh m = dev_mesh(c);
+    const DevPhys ph = dev_phys(c);
+    const int K = c->K;
+    const int face_blocks = (c->M * S::NF + BLOCK - 1) / BLOCK;
+    StageArgs st{0.0, 0.0, 0, 0, nullptr, nullptr};
+    TRY(run_rhs<P>(c, JQ, st, c->Qt, false, false, false));   // forward intermediates T, QA, FI, FV
+    hipStream_t s = c->stream;
+    double *nu_f = c->aNuPart, *nu_v = c->aNuPart + face_blocks;
+    return launch(c, KID_ADJ, [&] {
+        k_adj_update<P><<<K, BLOCK, 0, s>>>(Gb, Qb, cdt, c->aFTbe);
+        k_adj_face_visc<P><<<face_blocks, BLOCK, 0, s>>>(c->aFTbe, c->FV, m, ph, c->aFIb, c->aFVb, nu_f);
+        k_adj_grad_fwd<P><<<K, BLOCK, 0, s>>>(JQ, m, c->QA, c->aW);
+        k_adj_volume<P><<<K, BLOCK, 0, s>>>(JQ, m, ph, c->QA, nullptr, c->aW, Gb, c->aFVb, c->aQVb, c->aGVb, nu_v);
+        k_adj_gather<P><<<K, BLOCK, 0, s>>>(m, c->aQVb, c->aGVb, c->aJQvb, c->aQAbe);
+        k_adj_face_inv<P><<<face_blocks, BLOCK, 0, s>>>(c->T, m, ph, c->aFIb, c->aQAbe, c->aTb);
+        k_adj_traces<P><<<K, BLOCK, 0, s>>>(c->aTb, c->aJQvb, m, Out, Qb, Gb, a);
+        k_sum<<<1, BLOCK, 0, s>>>(c->aNuPart, face_blocks + K, nu_slot);
+    });
+}
+
+int rhs_vjp_dispatch(dsem_ctx *c, double *JQ, double *Gb, double *Qb, double cdt, double a, double *Out,
+                     double *nu_slot)
+{
+    DISPATCH_P(c, run_rhs_vjp, JQ, Gb, Qb, cdt, a, Out, nu_slot);
+}
+
 int upload_operators(dsem_ctx *c)
 {
     double D[9][90] = {}, B[9][90] = {}, Mo[9][81] = {}, cx[9][10] = {};
@@ -1189,6 +1406,22 @@ int upload_operators(dsem_ctx *c)
     HIP_TRY(c, hipMemcpyToSymbol(HIP_SYMBOL(c_B), B, sizeof(B)));
     HIP_TRY(c, hipMemcpyToSymbol(HIP_SYMBOL(c_M), Mo, sizeof(Mo)));
     HIP_TRY(c, hipMemcpyToSymbol(HIP_SYMBOL(c_cx), cx, sizeof(cx)));
+    float Df[9][90] = {}, Bf[9][90] = {}, Mf[9][81] = {};
+    double Dint[9][9] = {};
+    for (int p = 1; p <= 8; ++p) {
+        const int ncg = p + 1, nc = p + 2;
+        for (int i = 0; i < 90; ++i) { Df[p][i] = (float)D[p][i]; Bf[p][i] = (float)B[p][i]; }
+        for (int i = 0; i < 81; ++i) Mf[p][i] = (float)Mo[p][i];
+        for (int k = 0; k < ncg; ++k) {
+            double sum = 0.0;
+            for (int j = 1; j < nc - 1; ++j) sum += D[p][k * nc + j];
+            Dint[p][k] = sum;
+        }
+    }
+    HIP_TRY(c, hipMemcpyToSymbol(HIP_SYMBOL(c_Df), Df, sizeof(Df)));
+    HIP_TRY(c, hipMemcpyToSymbol(HIP_SYMBOL(c_Bf), Bf, sizeof(Bf)));
+    HIP_TRY(c, hipMemcpyToSymbol(HIP_SYMBOL(c_Mf), Mf, sizeof(Mf)));
+    HIP_TRY(c, hipMemcpyToSymbol(HIP_SYMBOL(c_Dint), Dint, sizeof(Dint)));
     return DSEM_OK;
 }
 
@@ -1198,8 +1431,8 @@ int create_impl(const dsem_mesh_desc *mesh, const dsem_params *prm, int device, 
     if (mesh->p < 1 || mesh->p > 8) return set_err(c, DSEM_ERR_ARG, "p must be in 1..8 (nc <= 10 as in size.f90)");
     if (mesh->K < 1 || mesh->nmortar < 1 || !mesh->mortar_elem || !mesh->mortar_face || !mesh->corners)
         return set_err(c, DSEM_ERR_ARG, "empty mesh");
-    if (prm->precision != DSEM_PREC_FP64)
-        return set_err(c, DSEM_ERR_UNSUPPORTED, "only the FP64 contraction path is built");
+    if (prm->precision != DSEM_PREC_FP64 && prm->precision != DSEM_PREC_FP32 && prm->precision != DSEM_PREC_BF16X3)
+        return set_err(c, DSEM_ERR_ARG, "precision must be DSEM_PREC_FP64, _FP32 or _BF16X3");
     if (!(prm->gamma > 1.0) || !(prm->re > 0.0) || !(prm->pr > 0.0) || !(prm->mach > 0.0))
         return set_err(c, DSEM_ERR_ARG, "physics parameters must be positive (gamma > 1)");
     c->device = device;
@@ -1556,6 +1789,114 @@ int dsem_rk_steps(dsem_ctx *c, double t, double dt, int nsteps, double *resid_ou
 int dsem_rk_step(dsem_ctx *c, double t, double dt, double *resid_out)
 {
     return dsem_rk_steps(c, t, dt, 1, resid_out);
+}
+
+static int vjp_check(dsem_ctx *c)
+{
+    if (!c || !c->stream) return set_err(c, DSEM_ERR_ARG, "vjp arguments");
+    if (c->params.shock)
+        return set_err(c, DSEM_ERR_UNSUPPORTED, "the adjoint covers the path without entropy viscosity (shock = 0)");
+    if (c->params.precision != DSEM_PREC_FP64)
+        return set_err(c, DSEM_ERR_UNSUPPORTED, "the adjoint is built for the FP64 path");
+    HIP_TRY(c, hipSetDevice(c->device));
+    return adjoint_alloc(c);
+}
+
+int dsem_rhs_vjp_device(dsem_ctx *c, const double *dJQ, const double *dLam, double *dJQbar, double *nubar)
+{
+    TRY(vjp_check(c));
+    if (!dJQ || !dLam || !dJQbar) return set_err(c, DSEM_ERR_ARG, "rhs_vjp_device pointers");
+    TRY(rhs_vjp_dispatch(c, const_cast<double *>(dJQ), const_cast<double *>(dLam), nullptr, 0.0, 0.0, dJQbar,
+                         c->aNu));
+    double h = 0.0;
+    HIP_TRY(c, hipMemcpyAsync(&h, c->aNu, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if (nubar) *nubar = h;
+    return DSEM_OK;
+}
+
+int dsem_rhs_vjp(dsem_ctx *c, const double *lam, double *JQbar, double *nubar)
+{
+    TRY(vjp_check(c));
+    if (!lam || !JQbar) return set_err(c, DSEM_ERR_ARG, "rhs_vjp pointers");
+    const size_t n = (size_t)c->K * NEQ * c->np;
+    HIP_TRY(c, hipMemcpyAsync(c->aLam, lam, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+    TRY(dsem_rhs_vjp_device(c, c->JQ, c->aLam, c->aOut, nubar));
+    HIP_TRY(c, hipMemcpyAsync(JQbar, c->aOut, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return DSEM_OK;
+}
+
+int dsem_rk_step_vjp_device(dsem_ctx *c, double dt, double *dQbar, double *nubar)
+{
+    TRY(vjp_check(c));
+    if (!dQbar) return set_err(c, DSEM_ERR_ARG, "rk_step_vjp_device pointer");
+    static const double ark[5] = {0.0, -0.4178904745, -1.192151694643, -1.697784692471, -1.514183444257};
+    static const double crk[5] = {0.1496590219993, 0.3792103129999, 0.8229550293869, 0.6994504559488, 0.1530572479681};
+    const size_t n = (size_t)c->K * NEQ * c->np;
+    double *Sg = c->aStages;
+    // stage input states S[0..4] by recomputing the step from the current state
+    HIP_TRY(c, hipMemcpyAsync(Sg, c->JQ, sizeof(double) * n, hipMemcpyDeviceToDevice, c->stream));
+    for (int k = 0; k < 4; ++k) {
+        HIP_TRY(c, hipMemcpyAsync(Sg + (k + 1) * n, Sg + k * n, sizeof(double) * n, hipMemcpyDeviceToDevice, c->stream));
+        StageArgs st;
+        st.a = ark[k];
+        st.cdt = crk[k] * dt;
+        st.first = (k == 0);
+        st.rk = 1;
+        st.resid = nullptr;
+        st.nonfinite = nullptr;
+        TRY(rhs_dispatch(c, Sg + (k + 1) * n, st, nullptr, false, false, false));
+    }
+    HIP_TRY(c, hipMemsetAsync(c->aGb, 0, sizeof(double) * n, c->stream));
+    for (int k = 4; k >= 0; --k)
+        TRY(rhs_vjp_dispatch(c, Sg + k * n, c->aGb, dQbar, crk[k] * dt, ark[k], nullptr, c->aNu + 1 + k));
+    double h[5];
+    HIP_TRY(c, hipMemcpyAsync(h, c->aNu + 1, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if (nubar) *nubar = h[4] + h[3] + h[2] + h[1] + h[0];
+    return DSEM_OK;
+}
+
+int dsem_rk_step_vjp(dsem_ctx *c, double dt, double *Qbar, double *nubar)
+{
+    TRY(vjp_check(c));
+    if (!Qbar) return set_err(c, DSEM_ERR_ARG, "rk_step_vjp pointer");
+    const size_t n = (size_t)c->K * NEQ * c->np;
+    HIP_TRY(c, hipMemcpyAsync(c->aOut, Qbar, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+    TRY(dsem_rk_step_vjp_device(c, dt, c->aOut, nubar));
+    HIP_TRY(c, hipMemcpyAsync(Qbar, c->aOut, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return DSEM_OK;
+}
+
+int dsem_set_state_device(dsem_ctx *c, const double *dJQ)
+{
+    if (!c || !c->stream || !dJQ) return set_err(c, DSEM_ERR_ARG, "set_state_device arguments");
+    HIP_TRY(c, hipSetDevice(c->device));
+    HIP_TRY(c, hipMemcpyAsync(c->JQ, dJQ, sizeof(double) * (size_t)c->K * NEQ * c->np, hipMemcpyDeviceToDevice,
+                              c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    c->have_lag = false;
+    return DSEM_OK;
+}
+
+int dsem_get_state_device(dsem_ctx *c, double *dJQ)
+{
+    if (!c || !c->stream || !dJQ) return set_err(c, DSEM_ERR_ARG, "get_state_device arguments");
+    HIP_TRY(c, hipSetDevice(c->device));
+    HIP_TRY(c, hipMemcpyAsync(dJQ, c->JQ, sizeof(double) * (size_t)c->K * NEQ * c->np, hipMemcpyDeviceToDevice,
+                              c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return DSEM_OK;
+}
+
+int dsem_quadrature_weights(const dsem_ctx *c, double *bmat)
+{
+    if (!c || !bmat) return DSEM_ERR_ARG;
+    const std::vector<double> w = dsem::ke_weights(c->ops);
+    std::memcpy(bmat, w.data(), sizeof(double) * w.size());
+    return DSEM_OK;
 }
 
 int dsem_dtcalc(dsem_ctx *c, double *dt_out)

@@ -169,6 +169,46 @@ class DSEM:
         self._check(self._L.dsem_rhs(self._h, t, _dp(out)))
         return out
 
+    # ---------------------------------------------------------------- adjoint
+    def rhs_vjp(self, lam: np.ndarray) -> tuple[np.ndarray, float]:
+        """((dR/dJQ)^T lam, <lam, dR/dnu>) at the current state, nu = 1/Re."""
+        lam = np.ascontiguousarray(lam, dtype=np.float64)
+        assert lam.shape == (self.K, 5, self.np)
+        out = np.zeros_like(lam)
+        nb = C.c_double(0.0)
+        self._check(self._L.dsem_rhs_vjp(self._h, _dp(lam), _dp(out), C.byref(nb)))
+        return out, nb.value
+
+    def rk_step_vjp(self, dt: float, qbar: np.ndarray) -> tuple[np.ndarray, float]:
+        """Adjoint of one step from the current state (left unchanged):
+        returns (adjoint of the start state, nu contribution)."""
+        q = np.array(qbar, dtype=np.float64, copy=True, order="C")
+        assert q.shape == (self.K, 5, self.np)
+        nb = C.c_double(0.0)
+        self._check(self._L.dsem_rk_step_vjp(self._h, dt, _dp(q), C.byref(nb)))
+        return q, nb.value
+
+    def rhs_vjp_device(self, jq_ptr: int, lam_ptr: int, out_ptr: int) -> float:
+        nb = C.c_double(0.0)
+        self._check(self._L.dsem_rhs_vjp_device(self._h, jq_ptr, lam_ptr, out_ptr, C.byref(nb)))
+        return nb.value
+
+    def rk_step_vjp_device(self, dt: float, qbar_ptr: int) -> float:
+        nb = C.c_double(0.0)
+        self._check(self._L.dsem_rk_step_vjp_device(self._h, dt, qbar_ptr, C.byref(nb)))
+        return nb.value
+
+    def set_state_device(self, ptr: int) -> None:
+        self._check(self._L.dsem_set_state_device(self._h, ptr))
+
+    def get_state_device(self, ptr: int) -> None:
+        self._check(self._L.dsem_get_state_device(self._h, ptr))
+
+    def quadrature_weights(self) -> np.ndarray:
+        w = np.zeros(self.np)
+        self._check(self._L.dsem_quadrature_weights(self._h, _dp(w)))
+        return w
+
     def tstep(self, time: float, dt: float) -> float:
         r = C.c_double(0.0)
         self._check(self._L.dsem_rk_step(self._h, time, dt, C.byref(r)))

@@ -157,6 +157,32 @@ int  dsem_dtcalc(dsem_ctx *ctx, double *dt_out);
  * Sums run per element, then over elements in a fixed order. */
 int  dsem_reduce_ke(dsem_ctx *ctx, double *Ek, double *eps);
 
+/* Adjoint (reverse mode) of the RHS and of the RK step, FP64, shock = 0.
+ * No counterpart in the reference, which has no derivative code: these
+ * differentiate the same discrete operators as dsem_rhs / dsem_rk_step
+ * (tstep, src/tstep.f90:12-120), with nu = 1/Re as a parameter (the RHS is
+ * linear in it).
+ * dsem_rhs_vjp: JQbar = (dR/dJQ)^T lam at the context's state, and
+ *   nubar = <lam, dR/dnu>; host buffers [K][5][ncg^3].
+ * dsem_rhs_vjp_device: the same at a device state dJQ, device buffers.
+ * dsem_rk_step_vjp: adjoint of one dsem_rk_step from the context's state
+ *   (the step's start state, left unchanged): Qbar holds the adjoint of the
+ *   state after the step on entry and of the state before it on return;
+ *   nubar receives this step's contribution.  The five stage states are
+ *   recomputed from the start state.  _device: dQbar on the device. */
+int  dsem_rhs_vjp(dsem_ctx *ctx, const double *lam, double *JQbar, double *nubar);
+int  dsem_rhs_vjp_device(dsem_ctx *ctx, const double *dJQ, const double *dLam, double *dJQbar, double *nubar);
+int  dsem_rk_step_vjp(dsem_ctx *ctx, double dt, double *Qbar, double *nubar);
+int  dsem_rk_step_vjp_device(dsem_ctx *ctx, double dt, double *dQbar, double *nubar);
+
+/* Device-to-device state transfer (checkpointing): dJQ [K][5][ncg^3]. */
+int  dsem_set_state_device(dsem_ctx *ctx, const double *dJQ);
+int  dsem_get_state_device(dsem_ctx *ctx, double *dJQ);
+
+/* Gauss-point quadrature weights on [0,1]^3 used by dsem_reduce_ke
+ * (bmatu1b, caseFiles/tgv/BackstepProblem-stoch.f90:345-361), [ncg^3]. */
+int  dsem_quadrature_weights(const dsem_ctx *ctx, double *bmat);
+
 /* Keep the last stage's face-averaged staggered states after every step
  * (d%Qlgg/Qglg/Qggl survive tstep in the reference) when entropy viscosity
  * is off, for the enstrophy diagnostic; costs one extra write of
