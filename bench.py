@@ -32,6 +32,13 @@ from dsem_amd import mesh as M  # noqa: E402
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
 
+# HBM bytes per launch measured with rocprofv3 on this workload (22^3, p=8,
+# FP64, per-point metrics): FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, the factors
+# that make the counters read 1 GiB for a 1 GiB device copy.  Source:
+# profiles/tgv22p8_fp64.md.
+MEASURED_TRAFFIC = {(22, 8, "fp64", "per_point"): {"k_volume": 2.368e9, "k_update": 1.970e9, "k_face_inv": 0.541e9,
+                                                   "k_face_visc": 0.478e9, "k_traces": 0.907e9}}
+
 
 def kernel_work(n_elem: int, nmort: int, p: int) -> dict:
     """Algorithmic bytes / flops per launch for every kernel (FP64).
@@ -86,6 +93,33 @@ def cpu_baseline(mesh, JQ, args) -> dict:
                       f"(OpenMP, {cores} threads), {dt:.1f} s"}
 
 
+def mixed_modes(mesh, JQ, s64, dt, args, ms64) -> dict:
+    """Throughput of the FP32 / BF16x3 contraction modes on the same input,
+    and their per-field relative L2 error of one RHS against the FP64 path."""
+    from dsem_amd.mixed_study import field_errors
+    from dsem_amd.solver import DSEM
+    s64.set_state(JQ)
+    r64 = s64.rhs()
+    res = {}
+    for mode in ("fp32", "bf16x3"):
+        with DSEM(mesh, re=1600.0, pr=0.71, mach=1.0, precision=mode, metrics=args.metrics) as sm:
+            sm.set_state(JQ)
+            err = field_errors(sm.rhs(), r64)
+            sm.steps(2, dt, check=False)
+            sm.synchronize()
+            t0 = time.perf_counter()
+            sm.steps(args.steps, dt, check=False)
+            sm.synchronize()
+            el = time.perf_counter() - t0
+        ms = el * 1e3 / args.steps
+        res[mode] = {"value": round(mesh.dof * 5 * args.steps / el / 1e9, 4), "unit": "GDOF/s per RHS",
+                     "ms_per_step": round(ms, 4), "speedup_vs_fp64": round(ms64 / ms, 3),
+                     "rel_l2_max_over_fields": max(v["rel_l2"] for v in err.values()),
+                     "rel_l2": {k: v["rel_l2"] for k, v in err.items()},
+                     "rel_linf": {k: v["rel_linf"] for k, v in err.items()}}
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -98,6 +132,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-roofline", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the FP32 / BF16x3 contraction modes")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint cost/gradient check (8^3, p=6)")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="run one device-to-device copy kernel of this many bytes first (counter calibration)")
     args = ap.parse_args()
@@ -173,17 +209,37 @@ def main():
             ach = wk["flops"] / avg_s / 1e12
             out["roofline"] = {"bound": "mfma", "kernel": dom, "achieved": round(ach, 3), "peak": FP64_PEAK_TFLOPS,
                                "unit": "TFLOP/s", "frac": round(ach / FP64_PEAK_TFLOPS, 4), "traffic": None,
-                               "note": "FP64 peak; algorithmic flops per launch = contraction + pointwise flux "
-                                       "flops (bench.kernel_work)"}
+                               "note": "FP64 peak (vector and matrix); algorithmic flops per launch = contraction "
+                                       "+ pointwise flux flops (bench.kernel_work); traffic = measured HBM bytes "
+                                       "per launch"}
         else:
             ach = wk["bytes"] / avg_s / 1e9
             out["roofline"] = {"bound": "hbm", "kernel": dom, "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
                                "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None}
+        meas = MEASURED_TRAFFIC.get((args.elements, args.p, args.precision, args.metrics))
+        if meas and dom in meas:
+            out["roofline"]["traffic"] = meas[dom]
+        for name in per:
+            if meas and name in meas:
+                per[name]["hbm_GB_measured"] = round(meas[name] / 1e9, 3)
+                per[name]["hbm_GB/s_measured"] = round(meas[name] / (per[name]["avg_ms"] * 1e-3) / 1e9, 1)
         out["kernels"] = per
 
+    if not args.no_mixed and args.precision == "fp64":
+        out["mixed"] = mixed_modes(mesh, JQ, s, dt, args, ms_step)
+    s.close()
+    if not args.no_adjoint:
+        from dsem_amd import adjoint_study
+        r = adjoint_study.run(n=8, p=6, steps=50, every=10, ndir=4)
+        out["adjoint"] = {"adjoint_over_forward": round(r["adjoint_over_forward"], 3),
+                          "forward_ms": round(r["forward_ms"], 2), "backward_ms": round(r["backward_ms"], 2),
+                          "grad_fd_max_rel_err": r["velocity_directions"]["max_rel_err"],
+                          "nu_fd_rel_err": r["nu"]["rel_err"],
+                          "config": "TGV 8^3 hex8, p=6, Re=1600, 50 steps, checkpoint every 10, "
+                                    "J = int 1/2 rho|u|^2 dV; dJ/du0 on 4 seeded directions and dJ/dnu vs "
+                                    "central differences"}
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(mesh, JQ, args)
-    s.close()
     print(json.dumps(out), flush=True)
 
 

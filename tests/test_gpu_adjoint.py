@@ -163,3 +163,16 @@ def test_adjoint_against_oracle_finite_differences():
         ad = float(np.sum(grad * d))
         print(f"adjoint {ad:.12e} oracle fd {fd:.12e} rel {_rel(ad, fd):.2e}", flush=True)
         assert _rel(ad, fd) < 1e-6
+
+
+def test_taylor_green_adjoint_gradient_8x8x8_p6():
+    """J = int 1/2 rho |u|^2 after 50 steps (8^3 hex8, p = 6, Re = 1600),
+    checkpoint every 10 steps: dJ/du0 along 32 seeded directions and dJ/dnu
+    against central differences."""
+    from dsem_amd import adjoint_study
+    print("\n[case] adjoint study 8^3 p=6 50 steps", flush=True)
+    r = adjoint_study.run(n=8, p=6, steps=50, every=10, ndir=32)
+    print({k: r[k] for k in ("forward_ms", "backward_ms", "adjoint_over_forward")}, r["velocity_directions"]["max_rel_err"],
+          r["nu"], flush=True)
+    assert r["velocity_directions"]["max_rel_err"] < 1e-6
+    assert r["nu"]["rel_err"] < 1e-6

@@ -1,0 +1,55 @@
+"""Mixed-precision contraction modes against the FP64 GPU path.
+
+FP32 contractions on element-shifted fields: one RHS of the throughput input
+(TGV with a seeded 1e-3 relative perturbation) within 1e-6 relative L2 per
+conserved field, and the state after 200 steps from the clean TGV field
+within 1e-6 relative L2 per field.  BF16x3 is checked against looser bounds
+(its split products keep ~16 significant bits).  The clean TGV RHS itself is
+dominated by low-Mach cancellation (the FP64 result carries ~1e-11 relative
+rounding noise), so its mixed-precision error is reported, not bounded.
+"""
+import numpy as np
+import pytest
+
+from dsem_amd import cases
+from dsem_amd import mesh as M
+from dsem_amd.mixed_study import field_errors
+from dsem_amd.solver import DSEM
+
+pytestmark = pytest.mark.gpu
+
+BOUNDS = {"fp32": (1e-6, 1e-6), "bf16x3": (2e-4, 1e-4)}
+
+
+@pytest.mark.parametrize("mode", ["fp32", "bf16x3"])
+def test_mixed_rhs_and_state_against_fp64(mode):
+    print(f"\n[case] mixed {mode} 22^3 p=8", flush=True)
+    mesh = M.periodic_box(22, 8)
+    rhs_bound, state_bound = BOUNDS[mode]
+    with DSEM(mesh, re=1600.0, pr=0.71, metrics="per_point") as s64, \
+            DSEM(mesh, re=1600.0, pr=0.71, precision=mode, metrics="per_point") as smx:
+        xg, jac = s64.gauss_points()
+        pert = cases.tgv_state(xg, jac, "reference", perturb=1e-3, seed=1234)
+        s64.set_state(pert)
+        smx.set_state(pert)
+        e = field_errors(smx.rhs(), s64.rhs())
+        print("rhs", {k: round(v["rel_l2"], 10) for k, v in e.items()}, flush=True)
+        assert all(v["rel_l2"] < rhs_bound for v in e.values())
+        clean = cases.tgv_state(xg, jac, "isothermal")
+        s64.set_state(clean)
+        dt = s64.dtcalc(0.5)
+        smx.set_state(clean)
+        s64.steps(200, dt)
+        smx.steps(200, dt)
+        e = field_errors(smx.get_state(), s64.get_state())
+        print("state", {k: round(v["rel_l2"], 12) for k, v in e.items()}, flush=True)
+        assert all(v["rel_l2"] < state_bound for v in e.values())
+
+
+def test_mixed_precision_is_rejected_for_adjoint():
+    mesh = M.periodic_box(2, 3)
+    with DSEM(mesh, precision="fp32") as s:
+        xg, jac = s.gauss_points()
+        s.set_state(cases.tgv_state(xg, jac))
+        with pytest.raises(Exception):
+            s.rhs_vjp(np.ones((s.K, 5, s.np)))
