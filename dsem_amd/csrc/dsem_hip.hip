@@ -339,7 +339,11 @@ __device__ __forceinline__ void load_physical_state(const double *__restrict__ J
 // One staggered grid D of the fused volume kernel: fluxes at the nc*ncg^2
 // points of grid D into LDS (interior) or the viscous face traces (faces),
 // then the interior part of the divergence along D added into acc.
-template <int P, int D, int PM>
+// NPT staggered points per work-item: with NPT = 2 a work-item takes the
+// points j and j + nc/2 of one line, so the 9 LDS values of the line are
+// loaded once for both (interpolation of Q and prolongation of the 15
+// gradients); the arithmetic of each point is unchanged.
+template <int P, int D, int PM, int NPT>
 __device__ __forceinline__ void volume_grid(const DevMesh &m, const DevPhys &ph, const double *__restrict__ QA,
                                             const double *__restrict__ muh, double *__restrict__ FV,
                                             double *__restrict__ Slag, const typename Ctr<PM>::T *sQ,
@@ -350,105 +354,149 @@ __device__ __forceinline__ void volume_grid(const DevMesh &m, const DevPhys &ph,
     using S = Sz<P>;
     using CT = typename Ctr<PM>::T;
     using CA = typename Ctr<PM>::A;
+    static_assert(S::NC % NPT == 0, "paired points need an even nc");
     constexpr int R = (S::NP + VBLOCK - 1) / VBLOCK;
+    constexpr int JH = S::NC / NPT;    // j distance between the points of a work-item
+    constexpr int NW = S::NS / NPT;    // work-items per grid
     const int tid = threadIdx.x;
     const int nsm = m.ms ? S::NS : 1, nfm = m.ms ? S::NF : 1;
     const double *gmd = (D == 0 ? m.gmet0 : (D == 1 ? m.gmet1 : m.gmet2)) + (size_t)e * 3 * nsm;
     __syncthreads();   // sG complete / previous divergence pass done with sF
 #pragma unroll 1
-    for (int s = tid; s < S::NS; s += VBLOCK) {
-        int j, a, b;
-        if (D == 0) { j = s % S::NC; a = (s / S::NC) % S::NCG; b = s / (S::NC * S::NCG); }
-        else if (D == 1) { a = s % S::NCG; j = (s / S::NCG) % S::NC; b = s / (S::NCG * S::NC); }
-        else { a = s % S::NCG; b = (s / S::NCG) % S::NCG; j = s / (S::NCG * S::NCG); }
-        const bool face = (j == 0 || j == S::NC - 1);
-        const int q = a + S::NCG * b;
-        CT Bj[S::NCG];
-#pragma unroll
-        for (int k = 0; k < S::NCG; ++k) Bj[k] = opB<PM>(P, j * S::NCG + k);
-        const int fc = face_of<P>(D, j);
-        const int mo = mort[fc - 1];
-        double Qv[5];
-        if (face) {
-#pragma unroll
-            for (int v = 0; v < 5; ++v) Qv[v] = QA[((size_t)mo * NEQ + v) * S::NF + q];
+    for (int w = tid; w < NW; w += VBLOCK) {
+        int a, b, j[NPT];
+        if constexpr (NPT == 1) {
+            const int s0 = w;
+            if (D == 0) { j[0] = s0 % S::NC; a = (s0 / S::NC) % S::NCG; b = s0 / (S::NC * S::NCG); }
+            else if (D == 1) { a = s0 % S::NCG; j[0] = (s0 / S::NCG) % S::NC; b = s0 / (S::NCG * S::NC); }
+            else { a = s0 % S::NCG; b = (s0 / S::NCG) % S::NCG; j[0] = s0 / (S::NCG * S::NCG); }
         } else {
+            const int line = w / JH, jj = w % JH;
+            a = line % S::NCG;
+            b = line / S::NCG;
 #pragma unroll
-            for (int v = 0; v < 5; ++v) {
-                CA t = 0;
+            for (int u = 0; u < NPT; ++u) j[u] = jj + u * JH;
+        }
+        const int q = a + S::NCG * b;
+        int s[NPT], fc[NPT], mo[NPT];
+        bool face[NPT];
+        CT Bj[NPT][S::NCG];
 #pragma unroll
-                for (int k = 0; k < S::NCG; ++k) t = mac<PM>(Bj[k], sQ[v * S::NP + gidx<P>(D, k, a, b)], t);
-                if constexpr (PM == 0) Qv[v] = t; else Qv[v] = sBar[v] + (double)t;
-                if (v == 1 || v == 3) __builtin_amdgcn_sched_barrier(0);
+        for (int u = 0; u < NPT; ++u) {
+            s[u] = sidx<P>(D, j[u], a, b);
+            face[u] = (j[u] == 0 || j[u] == S::NC - 1);
+            fc[u] = face_of<P>(D, j[u]);
+            mo[u] = mort[fc[u] - 1];
+#pragma unroll
+            for (int k = 0; k < S::NCG; ++k) Bj[u][k] = opB<PM>(P, j[u] * S::NCG + k);
+        }
+        double Qv[NPT][5];
+#pragma unroll
+        for (int v = 0; v < 5; ++v) {
+            CT ql[S::NCG];
+#pragma unroll
+            for (int k = 0; k < S::NCG; ++k) ql[k] = sQ[v * S::NP + gidx<P>(D, k, a, b)];
+#pragma unroll
+            for (int u = 0; u < NPT; ++u) {
+                if (face[u]) {
+                    Qv[u][v] = QA[((size_t)mo[u] * NEQ + v) * S::NF + q];
+                } else {
+                    CA t = 0;
+#pragma unroll
+                    for (int k = 0; k < S::NCG; ++k) t = mac<PM>(Bj[u][k], ql[k], t);
+                    if constexpr (PM == 0) Qv[u][v] = t; else Qv[u][v] = sBar[v] + (double)t;
+                }
             }
+            if (v == 1 || v == 3) __builtin_amdgcn_sched_barrier(0);
         }
         if (Slag) {
 #pragma unroll
-            for (int v = 0; v < 5; ++v) Slag[(((size_t)e * 3 + D) * NEQ + v) * S::NS + s] = Qv[v];
+            for (int u = 0; u < NPT; ++u)
+#pragma unroll
+                for (int v = 0; v < 5; ++v) Slag[(((size_t)e * 3 + D) * NEQ + v) * S::NS + s[u]] = Qv[u][v];
         }
-        const Prim pr = primitives(ph, Qv);
+        Prim pr[NPT];
+#pragma unroll
+        for (int u = 0; u < NPT; ++u) pr[u] = primitives(ph, Qv[u]);
         // Velocity and temperature gradients, one direction c at a time: the
         // prolonged conservative gradient dQ/dx_c (5 fields) is turned into
         // (du/dx_c, dv/dx_c, dw/dx_c, dT/dx_c) and folded into the symmetric
         // velocity-gradient sums, so no more than one direction's loads are live.
-        double Sxx = 0.0, Syy = 0.0, Szz = 0.0, Sxy = 0.0, Sxz = 0.0, Syz = 0.0, div = 0.0;
-        double Tx = 0.0, Ty = 0.0, Tz = 0.0;
+        double Sxx[NPT], Syy[NPT], Szz[NPT], Sxy[NPT], Sxz[NPT], Syz[NPT], div[NPT], Tx[NPT], Ty[NPT], Tz[NPT];
+#pragma unroll
+        for (int u = 0; u < NPT; ++u) {
+            Sxx[u] = Syy[u] = Szz[u] = Sxy[u] = Sxz[u] = Syz[u] = div[u] = 0.0;
+            Tx[u] = Ty[u] = Tz[u] = 0.0;
+        }
 #pragma unroll 1
         for (int c = 0; c < 3; ++c) {
-            double Gc[5];
+            double Gc[NPT][5];
 #pragma unroll
             for (int v = 0; v < 5; ++v) {
                 const CT *gl = sG + (c * 5 + v) * S::NP;
-                CA t = 0;
+                CT gv[S::NCG];
 #pragma unroll
-                for (int k = 0; k < S::NCG; ++k) t = mac<PM>(Bj[k], gl[gidx<P>(D, k, a, b)], t);
-                Gc[v] = (double)t;
+                for (int k = 0; k < S::NCG; ++k) gv[k] = gl[gidx<P>(D, k, a, b)];
+#pragma unroll
+                for (int u = 0; u < NPT; ++u) {
+                    CA t = 0;
+#pragma unroll
+                    for (int k = 0; k < S::NCG; ++k) t = mac<PM>(Bj[u][k], gv[k], t);
+                    Gc[u][v] = (double)t;
+                }
             }
-            double o[4];
-            prim_gradient(ph, pr, Gc, o);
-            const double uc = o[0], vc = o[1], wc = o[2];
-            div = div + (c == 0 ? uc : (c == 1 ? vc : wc));
-            Sxx += (c == 0) ? uc : 0.0;
-            Syy += (c == 1) ? vc : 0.0;
-            Szz += (c == 2) ? wc : 0.0;
-            Sxy += (c == 0) ? vc : ((c == 1) ? uc : 0.0);   // uy + vx
-            Sxz += (c == 0) ? wc : ((c == 2) ? uc : 0.0);   // wx + uz
-            Syz += (c == 1) ? wc : ((c == 2) ? vc : 0.0);   // wy + vz
-            Tx = (c == 0) ? o[3] : Tx;
-            Ty = (c == 1) ? o[3] : Ty;
-            Tz = (c == 2) ? o[3] : Tz;
-        }
-        double amu = 1.0;
-        if (muh) amu = amu + muh[((size_t)e * 3 + D) * S::NS + s];
-        const int sq = s * m.ms;
-        const double g0 = gmd[0 * nsm + sq], g1 = gmd[1 * nsm + sq], g2 = gmd[2 * nsm + sq];
-        double rv[5];
-        {
-            // Stokes stress and Fourier heat flux (x/y/z_vis_flux, PhysLib.f90:464-470)
-            const double hfac = (1.0 + (amu - 1.0) * ph.kap) * ph.hfac1;
-            const double d3 = div * (1.0 / 3.0);
-            const double txx = 2.0 * amu * (Sxx - d3), tyy = 2.0 * amu * (Syy - d3), tzz = 2.0 * amu * (Szz - d3);
-            const double txy = amu * Sxy, txz = amu * Sxz, tyz = amu * Syz;
-            const double f4 = pr.u * txx + pr.v * txy + pr.w * txz + hfac * Tx;
-            const double g4 = pr.v * tyy + pr.u * txy + pr.w * tyz + hfac * Ty;
-            const double h4 = pr.v * tyz + pr.u * txz + pr.w * tzz + hfac * Tz;
-            rv[0] = 0.0;
-            rv[1] = g0 * txx + g1 * txy + g2 * txz;
-            rv[2] = g0 * txy + g1 * tyy + g2 * tyz;
-            rv[3] = g0 * txz + g1 * tyz + g2 * tzz;
-            rv[4] = g0 * f4 + g1 * g4 + g2 * h4;
-        }
-        if (face) {
-            const double nrm = m.norm[(size_t)mo * nfm + q * m.ms];
 #pragma unroll
-            for (int v = 0; v < 5; ++v) FV[(((size_t)e * 6 + fc - 1) * NEQ + v) * S::NF + q] = rv[v] / nrm;
-        } else {
-            double fi[5];
-            euler_from_primitive(pr, Qv, g0, g1, g2, fi);
+            for (int u = 0; u < NPT; ++u) {
+                double o[4];
+                prim_gradient(ph, pr[u], Gc[u], o);
+                const double uc = o[0], vc = o[1], wc = o[2];
+                div[u] = div[u] + (c == 0 ? uc : (c == 1 ? vc : wc));
+                Sxx[u] += (c == 0) ? uc : 0.0;
+                Syy[u] += (c == 1) ? vc : 0.0;
+                Szz[u] += (c == 2) ? wc : 0.0;
+                Sxy[u] += (c == 0) ? vc : ((c == 1) ? uc : 0.0);   // uy + vx
+                Sxz[u] += (c == 0) ? wc : ((c == 2) ? uc : 0.0);   // wx + uz
+                Syz[u] += (c == 1) ? wc : ((c == 2) ? vc : 0.0);   // wy + vz
+                Tx[u] = (c == 0) ? o[3] : Tx[u];
+                Ty[u] = (c == 1) ? o[3] : Ty[u];
+                Tz[u] = (c == 2) ? o[3] : Tz[u];
+            }
+        }
 #pragma unroll
-            for (int v = 0; v < 5; ++v) {
-                if constexpr (PM == 0) sF[v * S::NS + s] = fi[v] - rv[v] * ph.inv_re;
-                else sF[v * S::NS + s] = (CT)((fi[v] - rv[v] * ph.inv_re) - sBar[5 + D * 5 + v]);
+        for (int u = 0; u < NPT; ++u) {
+            double amu = 1.0;
+            if (muh) amu = amu + muh[((size_t)e * 3 + D) * S::NS + s[u]];
+            const int sq = s[u] * m.ms;
+            const double g0 = gmd[0 * nsm + sq], g1 = gmd[1 * nsm + sq], g2 = gmd[2 * nsm + sq];
+            double rv[5];
+            {
+                // Stokes stress and Fourier heat flux (x/y/z_vis_flux, PhysLib.f90:464-470)
+                const double hfac = (1.0 + (amu - 1.0) * ph.kap) * ph.hfac1;
+                const double d3 = div[u] * (1.0 / 3.0);
+                const double txx = 2.0 * amu * (Sxx[u] - d3), tyy = 2.0 * amu * (Syy[u] - d3), tzz = 2.0 * amu * (Szz[u] - d3);
+                const double txy = amu * Sxy[u], txz = amu * Sxz[u], tyz = amu * Syz[u];
+                const Prim &p_ = pr[u];
+                const double f4 = p_.u * txx + p_.v * txy + p_.w * txz + hfac * Tx[u];
+                const double g4 = p_.v * tyy + p_.u * txy + p_.w * tyz + hfac * Ty[u];
+                const double h4 = p_.v * tyz + p_.u * txz + p_.w * tzz + hfac * Tz[u];
+                rv[0] = 0.0;
+                rv[1] = g0 * txx + g1 * txy + g2 * txz;
+                rv[2] = g0 * txy + g1 * tyy + g2 * tyz;
+                rv[3] = g0 * txz + g1 * tyz + g2 * tzz;
+                rv[4] = g0 * f4 + g1 * g4 + g2 * h4;
+            }
+            if (face[u]) {
+                const double nrm = m.norm[(size_t)mo[u] * nfm + q * m.ms];
+#pragma unroll
+                for (int v = 0; v < 5; ++v) FV[(((size_t)e * 6 + fc[u] - 1) * NEQ + v) * S::NF + q] = rv[v] / nrm;
+            } else {
+                double fi[5];
+                euler_from_primitive(pr[u], Qv[u], g0, g1, g2, fi);
+#pragma unroll
+                for (int v = 0; v < 5; ++v) {
+                    if constexpr (PM == 0) sF[v * S::NS + s[u]] = fi[v] - rv[v] * ph.inv_re;
+                    else sF[v * S::NS + s[u]] = (CT)((fi[v] - rv[v] * ph.inv_re) - sBar[5 + D * 5 + v]);
+                }
             }
         }
     }
@@ -591,9 +639,13 @@ __global__ __launch_bounds__(VBLOCK) void k_volume(const double *__restrict__ JQ
     for (int r = 0; r < R; ++r)
 #pragma unroll
         for (int v = 0; v < 5; ++v) acc[r][v] = 0.0;
-    volume_grid<P, 0, PM>(m, ph, QA, muh, FV, Slag, sQ, sG, sF, sBar, e, mort, acc);
-    volume_grid<P, 1, PM>(m, ph, QA, muh, FV, Slag, sQ, sG, sF, sBar, e, mort, acc);
-    volume_grid<P, 2, PM>(m, ph, QA, muh, FV, Slag, sQ, sG, sF, sBar, e, mort, acc);
+    // FP64: the element's LDS allows one workgroup per CU whatever the VGPR
+    // count, so two points per work-item cost no occupancy; the FP32 modes
+    // fit two workgroups per CU and keep one point per work-item.
+    constexpr int NPT = (PM == 0 && Sz<P>::NC % 2 == 0) ? 2 : 1;
+    volume_grid<P, 0, PM, NPT>(m, ph, QA, muh, FV, Slag, sQ, sG, sF, sBar, e, mort, acc);
+    volume_grid<P, 1, PM, NPT>(m, ph, QA, muh, FV, Slag, sQ, sG, sF, sBar, e, mort, acc);
+    volume_grid<P, 2, PM, NPT>(m, ph, QA, muh, FV, Slag, sQ, sG, sF, sBar, e, mort, acc);
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         const int pt = tid + r * VBLOCK;
