@@ -92,6 +92,9 @@ def load() -> C.CDLL:
         "dsem_rhs_vjp_device": ([vp, vp, vp, vp, _dp], C.c_int),
         "dsem_rk_step_vjp": ([vp, C.c_double, _dp, _dp], C.c_int),
         "dsem_rk_step_vjp_device": ([vp, C.c_double, vp, _dp], C.c_int),
+        "dsem_tape_reserve": ([vp, C.c_int], C.c_int),
+        "dsem_rk_step_record": ([vp, C.c_double, C.c_int], C.c_int),
+        "dsem_rk_step_vjp_tape": ([vp, C.c_double, C.c_int, vp, _dp], C.c_int),
         "dsem_set_state_device": ([vp, vp], C.c_int),
         "dsem_get_state_device": ([vp, vp], C.c_int),
         "dsem_quadrature_weights": ([vp, _dp], C.c_int),

@@ -4,13 +4,14 @@
                dsem_rhs_device, backward dsem_rhs_vjp_device).
 ``RKSolve`` -- ``nsteps`` RK4(5) steps as a differentiable function of the
                initial state and of nu = 1/Re (forward dsem_rk_steps,
-               backward dsem_rk_step_vjp_device step by step).  Checkpointing
-               is two-level: the state is kept every ``every`` steps in the
-               forward pass; the backward pass recomputes the step start
-               states of one segment from its checkpoint, then runs the step
-               adjoints of that segment in reverse, so memory holds
-               nsteps/every + every states instead of nsteps (each step
-               adjoint recomputes its own five stage states).
+               backward by step adjoints).  Checkpointing is two-level: the
+               forward pass keeps the state every ``every`` steps; the
+               backward pass re-runs one segment from its checkpoint while
+               recording each step's stage states and face intermediates on
+               the library's step tape (dsem_rk_step_record), then runs the
+               segment's step adjoints in reverse from the tape
+               (dsem_rk_step_vjp_tape).  Memory: nsteps/every checkpoints plus
+               ``every`` tape slots; work: one extra forward pass.
 
 Tensors are FP64 on the solver's GPU, shaped [K, 5, ncg^3].  The library
 works on its own HIP stream and synchronises it before returning; this
@@ -82,18 +83,14 @@ class RKSolve(torch.autograd.Function):
         torch.cuda.synchronize()
         nubar = 0.0
         starts = list(range(0, n, every))
+        solver.tape_reserve(min(every, n))
         for si in reversed(range(len(starts))):
             s0, s1 = starts[si], min(starts[si] + every, n)
-            states = [ctx.cps[si]]
             solver.set_state_device(ctx.cps[si].data_ptr())
-            for _ in range(s0 + 1, s1):
-                solver.steps(1, dt, check=False)
-                t = torch.empty_like(qbar)
-                solver.get_state_device(t.data_ptr())
-                states.append(t)
-            for s in reversed(range(s0, s1)):
-                solver.set_state_device(states[s - s0].data_ptr())
-                nubar += solver.rk_step_vjp_device(dt, qbar.data_ptr())
+            for i in range(s1 - s0):
+                solver.rk_step_record(dt, i)
+            for i in reversed(range(s1 - s0)):
+                nubar += solver.rk_step_vjp_tape(dt, i, qbar.data_ptr())
         return qbar, torch.tensor(nubar, dtype=torch.float64, device=qbar.device), None, None, None, None
 
 

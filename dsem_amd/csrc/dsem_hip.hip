@@ -1171,7 +1171,12 @@ struct dsem_ctx {
     // adjoint buffers (allocated by the first VJP call)
     double *aW = nullptr, *aQVb = nullptr, *aGVb = nullptr, *aFTbe = nullptr, *aFVb = nullptr, *aQAbe = nullptr;
     double *aTb = nullptr, *aFIb = nullptr, *aJQvb = nullptr, *aNuPart = nullptr, *aNu = nullptr;
-    double *aStages = nullptr, *aGb = nullptr, *aLam = nullptr, *aOut = nullptr;
+    double *aGb = nullptr, *aLam = nullptr, *aOut = nullptr;
+    // step tape: per slot and stage the stage input state and the forward
+    // intermediates the adjoint reads (face traces T, mortar averages QA,
+    // viscous face traces FV)
+    double *tape = nullptr;
+    int tape_slots = 0;
     int *e2m = nullptr, *m_elem = nullptr, *m_face = nullptr, *m_sign = nullptr, *m_nsign = nullptr;
     unsigned long long *d_scalars = nullptr;   // [0] resid bits, [1] dt bits, [2] nonfinite flag
     std::vector<void *> allocations;
@@ -1402,45 +1407,143 @@ int adjoint_alloc(dsem_ctx *c)
     TRY(dev_alloc(c, &c->aJQvb, n));
     TRY(dev_alloc(c, &c->aNuPart, K + face_blocks));
     TRY(dev_alloc(c, &c->aNu, 16));
-    TRY(dev_alloc(c, &c->aStages, 5 * n));
     TRY(dev_alloc(c, &c->aGb, n));
     TRY(dev_alloc(c, &c->aLam, n));
     TRY(dev_alloc(c, &c->aOut, n));
     return DSEM_OK;
 }
 
+// Forward intermediates of one RHS evaluation that its adjoint reads.
+struct FwdTape {
+    double *T, *QA, *FV;
+};
+
 // Adjoint of one RHS evaluation at state JQ (device) for the cotangent in
-// Gb.  Plain VJP (Qb == nullptr): Out = (dR/dJQ)^T Gb.  RK stage reverse
+// Gb, with the forward intermediates in f (recomputed first when fwd).
+// Plain VJP (Qb == nullptr): Out = (dR/dJQ)^T Gb.  RK stage reverse
 // (Qb != nullptr): Gb <- Gb + cdt Qb, Qb += (dR/dJQ)^T Gb, Gb <- a Gb.
 // nu_slot receives <Gb, dR/dnu> (device, one double).
 template <int P>
-int run_rhs_vjp(dsem_ctx *c, double *JQ, double *Gb, double *Qb, double cdt, double a, double *Out, double *nu_slot)
+int run_rhs_vjp(dsem_ctx *c, double *JQ, FwdTape f, bool fwd, double *Gb, double *Qb, double cdt, double a,
+                double *Out, double *nu_slot)
 {
     using S = Sz<P>;
     const DevMesh m = dev_mesh(c);
     const DevPhys ph = dev_phys(c);
     const int K = c->K;
     const int face_blocks = (c->M * S::NF + BLOCK - 1) / BLOCK;
-    StageArgs st{0.0, 0.0, 0, 0, nullptr, nullptr};
-    TRY(run_rhs<P>(c, JQ, st, c->Qt, false, false, false));   // forward intermediates T, QA, FI, FV
+    if (fwd) {
+        StageArgs st{0.0, 0.0, 0, 0, nullptr, nullptr};
+        TRY(run_rhs<P>(c, JQ, st, c->Qt, false, false, false));   // forward intermediates in c->T, QA, FV
+        f = FwdTape{c->T, c->QA, c->FV};
+    }
     hipStream_t s = c->stream;
     double *nu_f = c->aNuPart, *nu_v = c->aNuPart + face_blocks;
     return launch(c, KID_ADJ, [&] {
         k_adj_update<P><<<K, BLOCK, 0, s>>>(Gb, Qb, cdt, c->aFTbe);
-        k_adj_face_visc<P><<<face_blocks, BLOCK, 0, s>>>(c->aFTbe, c->FV, m, ph, c->aFIb, c->aFVb, nu_f);
-        k_adj_grad_fwd<P><<<K, BLOCK, 0, s>>>(JQ, m, c->QA, c->aW);
-        k_adj_volume<P><<<K, BLOCK, 0, s>>>(JQ, m, ph, c->QA, nullptr, c->aW, Gb, c->aFVb, c->aQVb, c->aGVb, nu_v);
+        k_adj_face_visc<P><<<face_blocks, BLOCK, 0, s>>>(c->aFTbe, f.FV, m, ph, c->aFIb, c->aFVb, nu_f);
+        k_adj_grad_fwd<P><<<K, BLOCK, 0, s>>>(JQ, m, f.QA, c->aW);
+        k_adj_volume<P><<<K, BLOCK, 0, s>>>(JQ, m, ph, f.QA, nullptr, c->aW, Gb, c->aFVb, c->aQVb, c->aGVb, nu_v);
         k_adj_gather<P><<<K, BLOCK, 0, s>>>(m, c->aQVb, c->aGVb, c->aJQvb, c->aQAbe);
-        k_adj_face_inv<P><<<face_blocks, BLOCK, 0, s>>>(c->T, m, ph, c->aFIb, c->aQAbe, c->aTb);
+        k_adj_face_inv<P><<<face_blocks, BLOCK, 0, s>>>(f.T, m, ph, c->aFIb, c->aQAbe, c->aTb);
         k_adj_traces<P><<<K, BLOCK, 0, s>>>(c->aTb, c->aJQvb, m, Out, Qb, Gb, a);
         k_sum<<<1, BLOCK, 0, s>>>(c->aNuPart, face_blocks + K, nu_slot);
     });
 }
 
-int rhs_vjp_dispatch(dsem_ctx *c, double *JQ, double *Gb, double *Qb, double cdt, double a, double *Out,
-                     double *nu_slot)
+int rhs_vjp_dispatch(dsem_ctx *c, double *JQ, FwdTape f, bool fwd, double *Gb, double *Qb, double cdt, double a,
+                     double *Out, double *nu_slot)
 {
-    DISPATCH_P(c, run_rhs_vjp, JQ, Gb, Qb, cdt, a, Out, nu_slot);
+    DISPATCH_P(c, run_rhs_vjp, JQ, f, fwd, Gb, Qb, cdt, a, Out, nu_slot);
+}
+
+// Tape layout per slot and stage: state [n], T [nT], QA [nQA], FV [nT].
+size_t tape_stage_size(const dsem_ctx *c)
+{
+    const size_t n = (size_t)c->K * NEQ * c->np, nT = (size_t)c->K * 6 * NEQ * c->nf;
+    const size_t nQA = (size_t)c->M * NEQ * c->nf;
+    return n + 2 * nT + nQA;
+}
+
+void tape_stage(const dsem_ctx *c, int slot, int k, double **S, FwdTape *f)
+{
+    const size_t n = (size_t)c->K * NEQ * c->np, nT = (size_t)c->K * 6 * NEQ * c->nf;
+    const size_t nQA = (size_t)c->M * NEQ * c->nf;
+    double *b = c->tape + ((size_t)slot * 5 + k) * tape_stage_size(c);
+    *S = b;
+    f->T = b + n;
+    f->QA = b + n + nT;
+    f->FV = b + n + nT + nQA;
+}
+
+int tape_reserve(dsem_ctx *c, int nslots)
+{
+    if (nslots <= c->tape_slots) return DSEM_OK;
+    if (c->tape) {
+        HIP_TRY(c, hipStreamSynchronize(c->stream));
+        HIP_TRY(c, hipFree(c->tape));
+        for (auto &p : c->allocations)
+            if (p == c->tape) p = nullptr;
+        c->tape = nullptr;
+        c->tape_slots = 0;
+    }
+    TRY(dev_alloc(c, &c->tape, (size_t)nslots * 5 * tape_stage_size(c)));
+    c->tape_slots = nslots;
+    return DSEM_OK;
+}
+
+static const double kArk[5] = {0.0, -0.4178904745, -1.192151694643, -1.697784692471, -1.514183444257};
+static const double kCrk[5] = {0.1496590219993, 0.3792103129999, 0.8229550293869, 0.6994504559488, 0.1530572479681};
+
+// One RK step of c->JQ (advanced in place) that records each stage's input
+// state and forward intermediates into tape slot `slot`.
+int record_step(dsem_ctx *c, double dt, int slot)
+{
+    const size_t n = (size_t)c->K * NEQ * c->np;
+    double *T0 = c->T, *QA0 = c->QA, *FV0 = c->FV;
+    int rc = DSEM_OK;
+    for (int k = 0; k < 5 && rc == DSEM_OK; ++k) {
+        double *S;
+        FwdTape f;
+        tape_stage(c, slot, k, &S, &f);
+        hipError_t e = hipMemcpyAsync(S, c->JQ, sizeof(double) * n, hipMemcpyDeviceToDevice, c->stream);
+        if (e != hipSuccess) { rc = set_err(c, DSEM_ERR_HIP, hipGetErrorString(e)); break; }
+        StageArgs st;
+        st.a = kArk[k];
+        st.cdt = kCrk[k] * dt;
+        st.first = (k == 0);
+        st.rk = 1;
+        st.resid = nullptr;
+        st.nonfinite = nullptr;
+        c->T = f.T;
+        c->QA = f.QA;
+        c->FV = f.FV;
+        rc = rhs_dispatch(c, c->JQ, st, nullptr, false, false, false);
+        c->T = T0;
+        c->QA = QA0;
+        c->FV = FV0;
+    }
+    return rc;
+}
+
+// Adjoint of the step recorded in `slot`: dQbar (device) from the adjoint of
+// the state after the step to that of the state before it; nu_host += the
+// step's <., dR/dnu>.
+int vjp_recorded_step(dsem_ctx *c, double dt, int slot, double *dQbar, double *nu_host)
+{
+    const size_t n = (size_t)c->K * NEQ * c->np;
+    HIP_TRY(c, hipMemsetAsync(c->aGb, 0, sizeof(double) * n, c->stream));
+    for (int k = 4; k >= 0; --k) {
+        double *S;
+        FwdTape f;
+        tape_stage(c, slot, k, &S, &f);
+        TRY(rhs_vjp_dispatch(c, S, f, false, c->aGb, dQbar, kCrk[k] * dt, kArk[k], nullptr, c->aNu + 1 + k));
+    }
+    double h[5];
+    HIP_TRY(c, hipMemcpyAsync(h, c->aNu + 1, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if (nu_host) *nu_host = h[4] + h[3] + h[2] + h[1] + h[0];
+    return DSEM_OK;
 }
 
 int upload_operators(dsem_ctx *c)
@@ -1858,8 +1961,8 @@ int dsem_rhs_vjp_device(dsem_ctx *c, const double *dJQ, const double *dLam, doub
 {
     TRY(vjp_check(c));
     if (!dJQ || !dLam || !dJQbar) return set_err(c, DSEM_ERR_ARG, "rhs_vjp_device pointers");
-    TRY(rhs_vjp_dispatch(c, const_cast<double *>(dJQ), const_cast<double *>(dLam), nullptr, 0.0, 0.0, dJQbar,
-                         c->aNu));
+    TRY(rhs_vjp_dispatch(c, const_cast<double *>(dJQ), FwdTape{nullptr, nullptr, nullptr}, true,
+                         const_cast<double *>(dLam), nullptr, 0.0, 0.0, dJQbar, c->aNu));
     double h = 0.0;
     HIP_TRY(c, hipMemcpyAsync(&h, c->aNu, sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
@@ -1883,31 +1986,37 @@ int dsem_rk_step_vjp_device(dsem_ctx *c, double dt, double *dQbar, double *nubar
 {
     TRY(vjp_check(c));
     if (!dQbar) return set_err(c, DSEM_ERR_ARG, "rk_step_vjp_device pointer");
-    static const double ark[5] = {0.0, -0.4178904745, -1.192151694643, -1.697784692471, -1.514183444257};
-    static const double crk[5] = {0.1496590219993, 0.3792103129999, 0.8229550293869, 0.6994504559488, 0.1530572479681};
-    const size_t n = (size_t)c->K * NEQ * c->np;
-    double *Sg = c->aStages;
-    // stage input states S[0..4] by recomputing the step from the current state
-    HIP_TRY(c, hipMemcpyAsync(Sg, c->JQ, sizeof(double) * n, hipMemcpyDeviceToDevice, c->stream));
-    for (int k = 0; k < 4; ++k) {
-        HIP_TRY(c, hipMemcpyAsync(Sg + (k + 1) * n, Sg + k * n, sizeof(double) * n, hipMemcpyDeviceToDevice, c->stream));
-        StageArgs st;
-        st.a = ark[k];
-        st.cdt = crk[k] * dt;
-        st.first = (k == 0);
-        st.rk = 1;
-        st.resid = nullptr;
-        st.nonfinite = nullptr;
-        TRY(rhs_dispatch(c, Sg + (k + 1) * n, st, nullptr, false, false, false));
-    }
-    HIP_TRY(c, hipMemsetAsync(c->aGb, 0, sizeof(double) * n, c->stream));
-    for (int k = 4; k >= 0; --k)
-        TRY(rhs_vjp_dispatch(c, Sg + k * n, c->aGb, dQbar, crk[k] * dt, ark[k], nullptr, c->aNu + 1 + k));
-    double h[5];
-    HIP_TRY(c, hipMemcpyAsync(h, c->aNu + 1, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+    TRY(tape_reserve(c, 1));
+    TRY(record_step(c, dt, 0));
+    double *S0;
+    FwdTape f;
+    tape_stage(c, 0, 0, &S0, &f);
+    HIP_TRY(c, hipMemcpyAsync(c->JQ, S0, sizeof(double) * (size_t)c->K * NEQ * c->np, hipMemcpyDeviceToDevice,
+                              c->stream));   // the start state stays the context's state
+    return vjp_recorded_step(c, dt, 0, dQbar, nubar);
+}
+
+int dsem_tape_reserve(dsem_ctx *c, int nslots)
+{
+    TRY(vjp_check(c));
+    if (nslots < 1) return set_err(c, DSEM_ERR_ARG, "tape_reserve: nslots >= 1");
+    return tape_reserve(c, nslots);
+}
+
+int dsem_rk_step_record(dsem_ctx *c, double dt, int slot)
+{
+    TRY(vjp_check(c));
+    if (slot < 0 || slot >= c->tape_slots) return set_err(c, DSEM_ERR_ARG, "rk_step_record: slot out of range");
+    TRY(record_step(c, dt, slot));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
-    if (nubar) *nubar = h[4] + h[3] + h[2] + h[1] + h[0];
     return DSEM_OK;
+}
+
+int dsem_rk_step_vjp_tape(dsem_ctx *c, double dt, int slot, double *dQbar, double *nubar)
+{
+    TRY(vjp_check(c));
+    if (slot < 0 || slot >= c->tape_slots || !dQbar) return set_err(c, DSEM_ERR_ARG, "rk_step_vjp_tape arguments");
+    return vjp_recorded_step(c, dt, slot, dQbar, nubar);
 }
 
 int dsem_rk_step_vjp(dsem_ctx *c, double dt, double *Qbar, double *nubar)

@@ -198,6 +198,17 @@ class DSEM:
         self._check(self._L.dsem_rk_step_vjp_device(self._h, dt, qbar_ptr, C.byref(nb)))
         return nb.value
 
+    def tape_reserve(self, nslots: int) -> None:
+        self._check(self._L.dsem_tape_reserve(self._h, nslots))
+
+    def rk_step_record(self, dt: float, slot: int) -> None:
+        self._check(self._L.dsem_rk_step_record(self._h, dt, slot))
+
+    def rk_step_vjp_tape(self, dt: float, slot: int, qbar_ptr: int) -> float:
+        nb = C.c_double(0.0)
+        self._check(self._L.dsem_rk_step_vjp_tape(self._h, dt, slot, qbar_ptr, C.byref(nb)))
+        return nb.value
+
     def set_state_device(self, ptr: int) -> None:
         self._check(self._L.dsem_set_state_device(self._h, ptr))
 

@@ -175,6 +175,17 @@ int  dsem_rhs_vjp_device(dsem_ctx *ctx, const double *dJQ, const double *dLam, d
 int  dsem_rk_step_vjp(dsem_ctx *ctx, double dt, double *Qbar, double *nubar);
 int  dsem_rk_step_vjp_device(dsem_ctx *ctx, double dt, double *dQbar, double *nubar);
 
+/* Step tape for checkpointed adjoints (no reference counterpart).
+ * dsem_tape_reserve allocates nslots slots, each holding one step's five
+ * stage input states and the forward face intermediates the adjoint reads
+ * (5 * (K*5*ncg^3 + 2*K*30*ncg^2 + nmortar*5*ncg^2) doubles per slot).
+ * dsem_rk_step_record takes one step of the context's state (advancing it,
+ * as dsem_rk_step) and records it in `slot`; dsem_rk_step_vjp_tape is the
+ * adjoint of the step recorded there (dQbar on the device, in/out). */
+int  dsem_tape_reserve(dsem_ctx *ctx, int nslots);
+int  dsem_rk_step_record(dsem_ctx *ctx, double dt, int slot);
+int  dsem_rk_step_vjp_tape(dsem_ctx *ctx, double dt, int slot, double *dQbar, double *nubar);
+
 /* Device-to-device state transfer (checkpointing): dJQ [K][5][ncg^3]. */
 int  dsem_set_state_device(dsem_ctx *ctx, const double *dJQ);
 int  dsem_get_state_device(dsem_ctx *ctx, double *dJQ);
