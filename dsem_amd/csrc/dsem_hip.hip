@@ -1169,7 +1169,7 @@ struct dsem_ctx {
     bool keep_lag = false;                     // save the last stage's staggered states without EV
     double *bmat = nullptr, *diag_part = nullptr, *diag_out = nullptr;   // diagnostics
     // adjoint buffers (allocated by the first VJP call)
-    double *aW = nullptr, *aQVb = nullptr, *aGVb = nullptr, *aFTbe = nullptr, *aFVb = nullptr, *aQAbe = nullptr;
+    double *aQVb = nullptr, *aGVb = nullptr, *aFTbe = nullptr, *aFVb = nullptr, *aQAbe = nullptr;
     double *aTb = nullptr, *aFIb = nullptr, *aJQvb = nullptr, *aNuPart = nullptr, *aNu = nullptr;
     double *aGb = nullptr, *aLam = nullptr, *aOut = nullptr;
     // step tape: per slot and stage the stage input state and the forward
@@ -1393,10 +1393,9 @@ int diag_dispatch(dsem_ctx *c, bool vorticity)
 
 int adjoint_alloc(dsem_ctx *c)
 {
-    if (c->aW) return DSEM_OK;
+    if (c->aQVb) return DSEM_OK;
     const size_t K = c->K, n = K * NEQ * c->np, nfe = K * 6 * NEQ * c->nf;
     const size_t face_blocks = ((size_t)c->M * c->nf + BLOCK - 1) / BLOCK;
-    TRY(dev_alloc(c, &c->aW, K * 15 * c->np));
     TRY(dev_alloc(c, &c->aQVb, K * 3 * NEQ * c->ns));
     TRY(dev_alloc(c, &c->aGVb, K * 3 * 15 * c->ns));
     TRY(dev_alloc(c, &c->aFTbe, nfe));
@@ -1442,8 +1441,7 @@ int run_rhs_vjp(dsem_ctx *c, double *JQ, FwdTape f, bool fwd, double *Gb, double
     return launch(c, KID_ADJ, [&] {
         k_adj_update<P><<<K, BLOCK, 0, s>>>(Gb, Qb, cdt, c->aFTbe);
         k_adj_face_visc<P><<<face_blocks, BLOCK, 0, s>>>(c->aFTbe, f.FV, m, ph, c->aFIb, c->aFVb, nu_f);
-        k_adj_grad_fwd<P><<<K, BLOCK, 0, s>>>(JQ, m, f.QA, c->aW);
-        k_adj_volume<P><<<K, BLOCK, 0, s>>>(JQ, m, ph, f.QA, nullptr, c->aW, Gb, c->aFVb, c->aQVb, c->aGVb, nu_v);
+        k_adj_volume<P><<<K, ABLOCK, 0, s>>>(JQ, m, ph, f.QA, nullptr, Gb, c->aFVb, c->aQVb, c->aGVb, nu_v);
         k_adj_gather<P><<<K, BLOCK, 0, s>>>(m, c->aQVb, c->aGVb, c->aJQvb, c->aQAbe);
         k_adj_face_inv<P><<<face_blocks, BLOCK, 0, s>>>(f.T, m, ph, c->aFIb, c->aQAbe, c->aTb);
         k_adj_traces<P><<<K, BLOCK, 0, s>>>(c->aTb, c->aJQvb, m, Out, Qb, Gb, a);
