@@ -88,6 +88,9 @@ def load() -> C.CDLL:
         "dsem_get_lagged_states": ([vp, _dp], C.c_int),
         "dsem_reduce_ke": ([vp, _dp, _dp], C.c_int),
         "dsem_keep_lagged_states": ([vp, C.c_int], C.c_int),
+        "dsem_set_les": ([vp, C.c_int, C.c_double, C.c_double], C.c_int),
+        "dsem_les_update": ([vp, _dp, _dp], C.c_int),
+        "dsem_set_les_viscosity": ([vp, _dp], C.c_int),
         "dsem_rhs_vjp": ([vp, _dp, _dp, _dp], C.c_int),
         "dsem_rhs_vjp_device": ([vp, vp, vp, vp, _dp], C.c_int),
         "dsem_rk_step_vjp": ([vp, C.c_double, _dp, _dp], C.c_int),

@@ -345,8 +345,9 @@ __device__ __forceinline__ void load_physical_state(const double *__restrict__ J
 // gradients); the arithmetic of each point is unchanged.
 template <int P, int D, int PM, int NPT>
 __device__ __forceinline__ void volume_grid(const DevMesh &m, const DevPhys &ph, const double *__restrict__ QA,
-                                            const double *__restrict__ muh, double *__restrict__ FV,
-                                            double *__restrict__ Slag, const typename Ctr<PM>::T *sQ,
+                                            const double *__restrict__ muh, const double *__restrict__ nut,
+                                            double *__restrict__ FV, double *__restrict__ Slag,
+                                            const typename Ctr<PM>::T *sQ,
                                             const typename Ctr<PM>::T *sG, typename Ctr<PM>::T *sF,
                                             const double *sBar, int e, const int *mort,
                                             double (&acc)[(Sz<P>::NP + VBLOCK - 1) / VBLOCK][5])
@@ -465,6 +466,7 @@ __device__ __forceinline__ void volume_grid(const DevMesh &m, const DevPhys &ph,
 #pragma unroll
         for (int u = 0; u < NPT; ++u) {
             double amu = 1.0;
+            if (nut) amu = amu + nut[((size_t)e * 3 + D) * S::NS + s[u]] * Qv[u][0];   // static Smagorinsky
             if (muh) amu = amu + muh[((size_t)e * 3 + D) * S::NS + s[u]];
             const int sq = s[u] * m.ms;
             const double g0 = gmd[0 * nsm + sq], g1 = gmd[1 * nsm + sq], g2 = gmd[2 * nsm + sq];
@@ -543,8 +545,8 @@ __device__ __forceinline__ void volume_grid(const DevMesh &m, const DevPhys &ph,
 template <int P, int PM>
 __global__ __launch_bounds__(VBLOCK) void k_volume(const double *__restrict__ JQ, DevMesh m, DevPhys ph,
                                                     const double *__restrict__ QA, const double *__restrict__ muh,
-                                                    double *__restrict__ FV, double *__restrict__ Qtp,
-                                                    double *__restrict__ Slag)
+                                                    const double *__restrict__ nut, double *__restrict__ FV,
+                                                    double *__restrict__ Qtp, double *__restrict__ Slag)
 {
     using S = Sz<P>;
     using CT = typename Ctr<PM>::T;
@@ -643,9 +645,9 @@ __global__ __launch_bounds__(VBLOCK) void k_volume(const double *__restrict__ JQ
     // count, so two points per work-item cost no occupancy; the FP32 modes
     // fit two workgroups per CU and keep one point per work-item.
     constexpr int NPT = (PM == 0 && Sz<P>::NC % 2 == 0) ? 2 : 1;
-    volume_grid<P, 0, PM, NPT>(m, ph, QA, muh, FV, Slag, sQ, sG, sF, sBar, e, mort, acc);
-    volume_grid<P, 1, PM, NPT>(m, ph, QA, muh, FV, Slag, sQ, sG, sF, sBar, e, mort, acc);
-    volume_grid<P, 2, PM, NPT>(m, ph, QA, muh, FV, Slag, sQ, sG, sF, sBar, e, mort, acc);
+    volume_grid<P, 0, PM, NPT>(m, ph, QA, muh, nut, FV, Slag, sQ, sG, sF, sBar, e, mort, acc);
+    volume_grid<P, 1, PM, NPT>(m, ph, QA, muh, nut, FV, Slag, sQ, sG, sF, sBar, e, mort, acc);
+    volume_grid<P, 2, PM, NPT>(m, ph, QA, muh, nut, FV, Slag, sQ, sG, sF, sBar, e, mort, acc);
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         const int pt = tid + r * VBLOCK;
@@ -995,6 +997,83 @@ __global__ __launch_bounds__(BLOCK) void k_ev(const double *__restrict__ JQ, con
     }
 }
 
+// Static Smagorinsky eddy viscosity (simpleSmag, staticSmag.f90:32-103;
+// filteredStrain :126-179; delta_g :191-201; nut_prolong :206-261) for one
+// element: Gauss-point gradients of the lagged staggered states,
+// S = (2 S_ij S_ij)^(1/2), nu_t = Re (C_s Delta)^2 S clamped to [0, numax],
+// prolonged to the three staggered grids with negative values set to zero.
+// As in filteredStrain, rho = (JQ)_1 jacob of the element's first point and
+// u = (JQ)_2 / rho.
+template <int P>
+__global__ __launch_bounds__(BLOCK) void k_smag(const double *__restrict__ JQ, const double *__restrict__ Slag,
+                                                 DevMesh m, DevPhys ph, double cs, double numax,
+                                                 double *__restrict__ nut, double *__restrict__ nut_g)
+{
+    using S = Sz<P>;
+    __shared__ double sS[3 * 4 * S::NS];
+    __shared__ double sNu[S::NP];
+    const int e = blockIdx.x, tid = threadIdx.x;
+    const double *src = Slag + (size_t)e * 3 * NEQ * S::NS;
+    for (int i = tid; i < 3 * 4 * S::NS; i += BLOCK) {
+        const int d = i / (4 * S::NS), r = i % (4 * S::NS);
+        sS[i] = src[d * NEQ * S::NS + r];
+    }
+    __syncthreads();
+    const int npm = m.ms ? S::NP : 1;
+    const double *gg = m.gmetg + (size_t)e * 9 * npm;
+    const double *jq = JQ + (size_t)e * NEQ * S::NP;
+    const double *jac = m.jacob + (size_t)e * npm;
+    const double c3 = pow((double)(S::NCG * S::NCG * S::NCG), 1.0 / 3.0);
+    for (int pt = tid; pt < S::NP; pt += BLOCK) {
+        const int n = pt % S::NCG, mi = (pt / S::NCG) % S::NCG, l = pt / (S::NCG * S::NCG);
+        const int pq = pt * m.ms;
+        double gr[4][3];
+#pragma unroll
+        for (int f = 0; f < 4; ++f) {
+            double wd[3];
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                const int k = d == 0 ? n : (d == 1 ? mi : l);
+                const int a = d == 0 ? mi : n;
+                const int b = d == 2 ? mi : l;
+                const double *fld = sS + (d * 4 + f) * S::NS;
+                double t = 0.0;
+#pragma unroll
+                for (int j = 0; j < S::NC; ++j) t = fma(c_D[P][k * S::NC + j], fld[sidx<P>(d, j, a, b)], t);
+                wd[d] = t;
+            }
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+                gr[f][c] = gg[(0 * 3 + c) * npm + pq] * wd[0] + gg[(1 * 3 + c) * npm + pq] * wd[1] +
+                           gg[(2 * 3 + c) * npm + pq] * wd[2];
+        }
+        const double rho = jq[pt] * jac[0];
+        const double u = jq[S::NP + pt] / rho, v = jq[2 * S::NP + pt] / rho, w = jq[3 * S::NP + pt] / rho;
+        const double ux = (gr[1][0] - gr[0][0] * u) / rho, uy = (gr[1][1] - gr[0][1] * u) / rho, uz = (gr[1][2] - gr[0][2] * u) / rho;
+        const double vx = (gr[2][0] - gr[0][0] * v) / rho, vy = (gr[2][1] - gr[0][1] * v) / rho, vz = (gr[2][2] - gr[0][2] * v) / rho;
+        const double wx = (gr[3][0] - gr[0][0] * w) / rho, wy = (gr[3][1] - gr[0][1] * w) / rho, wz = (gr[3][2] - gr[0][2] * w) / rho;
+        const double S12 = 0.50 * (uy + vx), S13 = 0.50 * (uz + wx), S23 = 0.50 * (vz + wy);
+        const double Sbar = sqrt(2.0 * ux * ux + 2.0 * S12 * S12 + 2.0 * S13 * S13 + 2.0 * S12 * S12 + 2.0 * vy * vy +
+                                 2.0 * S23 * S23 + 2.0 * S13 * S13 + 2.0 * S23 * S23 + 2.0 * wz * wz);
+        const double delta = pow(fabs(1.0 / jac[pq]), 1.0 / 3.0) / c3;
+        const double nt = ph.re * (cs * delta) * (cs * delta) * Sbar;
+        sNu[pt] = fmin(fmax(nt, 0.0), numax);
+        if (nut_g) nut_g[(size_t)e * S::NP + pt] = sNu[pt];
+    }
+    __syncthreads();
+    for (int i = tid; i < 3 * S::NS; i += BLOCK) {
+        const int d = i / S::NS, s2 = i % S::NS;
+        int j, a, b;
+        if (d == 0) { j = s2 % S::NC; a = (s2 / S::NC) % S::NCG; b = s2 / (S::NC * S::NCG); }
+        else if (d == 1) { a = s2 % S::NCG; j = (s2 / S::NCG) % S::NC; b = s2 / (S::NCG * S::NC); }
+        else { a = s2 % S::NCG; b = (s2 / S::NCG) % S::NCG; j = s2 / (S::NCG * S::NCG); }
+        double t = 0.0;
+#pragma unroll
+        for (int k = 0; k < S::NCG; ++k) t = fma(c_B[P][j * S::NCG + k], sNu[gidx<P>(d, k, a, b)], t);
+        nut[(size_t)e * 3 * S::NS + i] = t < 0.0 ? 0.0 : t;
+    }
+}
+
 // Per-element partial sums of the flow diagnostics, part[e][0..4]:
 //   0..2  sum (u^2, v^2, w^2) bmat over the Gauss points, u = (JQ)_2/(JQ)_1
 //         (turbulent kinetic energy of do_user_stuff,
@@ -1138,9 +1217,9 @@ __global__ __launch_bounds__(BLOCK) void k_dtcalc(const double *__restrict__ JQ,
 // ====================================================================== host
 namespace {
 
-enum KernelId { KID_TRACES = 0, KID_FACE_INV, KID_VOLUME, KID_FACE_VISC, KID_UPDATE, KID_DTCALC, KID_EV, KID_DIAG, KID_ADJ, KID_COUNT };
+enum KernelId { KID_TRACES = 0, KID_FACE_INV, KID_VOLUME, KID_FACE_VISC, KID_UPDATE, KID_DTCALC, KID_EV, KID_DIAG, KID_ADJ, KID_LES, KID_COUNT };
 const char *const kKernelNames[KID_COUNT] = {"k_traces", "k_face_inv", "k_volume", "k_face_visc",
-                                             "k_update", "k_dtcalc", "k_ev", "k_diag", "k_adj"};
+                                             "k_update", "k_dtcalc", "k_ev", "k_diag", "k_adj", "k_smag"};
 
 template <typename T>
 struct DevBuf {
@@ -1167,6 +1246,9 @@ struct dsem_ctx {
     double *Slag = nullptr, *muhg = nullptr, *ev_resid = nullptr, *ev_ducros = nullptr;   // entropy viscosity
     bool have_lag = false;
     bool keep_lag = false;                     // save the last stage's staggered states without EV
+    bool les = false;                          // static Smagorinsky model
+    double les_cs = 0.0, les_numax = 0.0;
+    double *nut = nullptr, *nut_g = nullptr;   // eddy viscosity on the staggered grids / Gauss points
     double *bmat = nullptr, *diag_part = nullptr, *diag_out = nullptr;   // diagnostics
     // adjoint buffers (allocated by the first VJP call)
     double *aQVb = nullptr, *aGVb = nullptr, *aFTbe = nullptr, *aFVb = nullptr, *aQAbe = nullptr;
@@ -1287,17 +1369,18 @@ int run_rhs(dsem_ctx *c, double *JQ, const StageArgs &st, double *Qt_out, bool t
     const int K = c->K;
     const int face_blocks = (c->M * S::NF + BLOCK - 1) / BLOCK;
     const double *muh = c->params.shock ? c->muh : nullptr;
+    const double *nut = c->les ? c->nut : nullptr;
     hipStream_t s = c->stream;
     if (!traces_ready) TRY(launch(c, KID_TRACES, [&] { k_traces<P><<<K, BLOCK, 0, s>>>(JQ, m, c->T); }));
     TRY(launch(c, KID_FACE_INV, [&] { k_face_inv<P><<<face_blocks, BLOCK, 0, s>>>(c->T, m, ph, c->FI, c->QA); }));
     double *slag = save_lag ? c->Slag : nullptr;
     TRY(launch(c, KID_VOLUME, [&] {
         if (c->params.precision == DSEM_PREC_FP32)
-            k_volume<P, 1><<<K, VBLOCK, 0, s>>>(JQ, m, ph, c->QA, muh, c->FV, c->Qtp, slag);
+            k_volume<P, 1><<<K, VBLOCK, 0, s>>>(JQ, m, ph, c->QA, muh, nut, c->FV, c->Qtp, slag);
         else if (c->params.precision == DSEM_PREC_BF16X3)
-            k_volume<P, 2><<<K, VBLOCK, 0, s>>>(JQ, m, ph, c->QA, muh, c->FV, c->Qtp, slag);
+            k_volume<P, 2><<<K, VBLOCK, 0, s>>>(JQ, m, ph, c->QA, muh, nut, c->FV, c->Qtp, slag);
         else
-            k_volume<P, 0><<<K, VBLOCK, 0, s>>>(JQ, m, ph, c->QA, muh, c->FV, c->Qtp, slag);
+            k_volume<P, 0><<<K, VBLOCK, 0, s>>>(JQ, m, ph, c->QA, muh, nut, c->FV, c->Qtp, slag);
     }));
     TRY(launch(c, KID_FACE_VISC, [&] { k_face_visc<P><<<face_blocks, BLOCK, 0, s>>>(c->FV, c->FI, m, ph, c->FT); }));
     if (make_traces)
@@ -1368,6 +1451,36 @@ int ev_step(dsem_ctx *c, bool debug)
         return DSEM_OK;
     }
     return ev_dispatch(c, debug);
+}
+
+template <int P>
+int run_les(dsem_ctx *c, bool gauss)
+{
+    const DevMesh m = dev_mesh(c);
+    const DevPhys ph = dev_phys(c);
+    return launch(c, KID_LES, [&] {
+        k_smag<P><<<c->K, BLOCK, 0, c->stream>>>(c->JQ, c->Slag, m, ph, c->les_cs, c->les_numax, c->nut,
+                                                 gauss ? c->nut_g : nullptr);
+    });
+}
+
+int les_dispatch(dsem_ctx *c, bool gauss)
+{
+    DISPATCH_P(c, run_les, gauss);
+}
+
+// Static Smagorinsky at the start of a step (tstep.f90:40-43): nu_t from the
+// lagged states, or zero on a fresh start (the reference's zero lagged
+// density gives 0/0 there, which its max(.,0) turns into 0).
+int les_step(dsem_ctx *c, bool gauss)
+{
+    if (!c->les) return DSEM_OK;
+    if (!c->have_lag) {
+        HIP_TRY(c, hipMemsetAsync(c->nut, 0, sizeof(double) * (size_t)c->K * 3 * c->ns, c->stream));
+        HIP_TRY(c, hipMemsetAsync(c->nut_g, 0, sizeof(double) * (size_t)c->K * c->np, c->stream));
+        return DSEM_OK;
+    }
+    return les_dispatch(c, gauss);
 }
 
 int dtcalc_dispatch(dsem_ctx *c, unsigned long long *out)
@@ -1856,6 +1969,50 @@ int dsem_keep_lagged_states(dsem_ctx *c, int enable)
     return DSEM_OK;
 }
 
+int dsem_set_les(dsem_ctx *c, int enable, double cs, double numax)
+{
+    if (!c || !c->stream) return set_err(c, DSEM_ERR_ARG, "set_les arguments");
+    if (enable && !(cs >= 0.0 && numax >= 0.0)) return set_err(c, DSEM_ERR_ARG, "set_les: cs and numax must be >= 0");
+    HIP_TRY(c, hipSetDevice(c->device));
+    if (enable) {
+        if (!c->Slag) TRY(dev_alloc(c, &c->Slag, (size_t)c->K * 3 * NEQ * c->ns));
+        if (!c->nut) {
+            TRY(dev_alloc(c, &c->nut, (size_t)c->K * 3 * c->ns));
+            TRY(dev_alloc(c, &c->nut_g, (size_t)c->K * c->np));
+            HIP_TRY(c, hipMemsetAsync(c->nut, 0, sizeof(double) * (size_t)c->K * 3 * c->ns, c->stream));
+            HIP_TRY(c, hipMemsetAsync(c->nut_g, 0, sizeof(double) * (size_t)c->K * c->np, c->stream));
+            HIP_TRY(c, hipStreamSynchronize(c->stream));
+        }
+        c->les_cs = cs;
+        c->les_numax = numax;
+    }
+    c->les = enable != 0;
+    return DSEM_OK;
+}
+
+int dsem_les_update(dsem_ctx *c, double *nut_g, double *nut_s)
+{
+    if (!c || !c->stream || !c->les) return set_err(c, DSEM_ERR_ARG, "les_update needs dsem_set_les(ctx, 1, ...)");
+    HIP_TRY(c, hipSetDevice(c->device));
+    TRY(les_step(c, true));
+    if (nut_g) HIP_TRY(c, hipMemcpyAsync(nut_g, c->nut_g, sizeof(double) * (size_t)c->K * c->np, hipMemcpyDeviceToHost,
+                                         c->stream));
+    if (nut_s) HIP_TRY(c, hipMemcpyAsync(nut_s, c->nut, sizeof(double) * (size_t)c->K * 3 * c->ns,
+                                         hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return DSEM_OK;
+}
+
+int dsem_set_les_viscosity(dsem_ctx *c, const double *nut_s)
+{
+    if (!c || !c->stream || !c->les || !nut_s) return set_err(c, DSEM_ERR_ARG, "set_les_viscosity needs the model on");
+    HIP_TRY(c, hipSetDevice(c->device));
+    HIP_TRY(c, hipMemcpyAsync(c->nut, nut_s, sizeof(double) * (size_t)c->K * 3 * c->ns, hipMemcpyHostToDevice,
+                              c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return DSEM_OK;
+}
+
 int dsem_reduce_ke(dsem_ctx *c, double *Ek, double *eps)
 {
     if (!c || !c->stream || (!Ek && !eps)) return set_err(c, DSEM_ERR_ARG, "reduce_ke arguments");
@@ -1910,6 +2067,7 @@ int dsem_rk_steps(dsem_ctx *c, double t, double dt, int nsteps, double *resid_ou
     if (check) HIP_TRY(c, hipMemsetAsync(c->d_scalars, 0, sizeof(unsigned long long) * 4, c->stream));
     for (int s = 0; s < nsteps; ++s) {
         TRY(ev_step(c, false));
+        TRY(les_step(c, false));
         for (int k = 0; k < 5; ++k) {
             StageArgs st;
             st.a = ark[k];
@@ -1921,9 +2079,9 @@ int dsem_rk_steps(dsem_ctx *c, double t, double dt, int nsteps, double *resid_ou
             st.nonfinite = check ? reinterpret_cast<unsigned int *>(&c->d_scalars[2]) : nullptr;
             const bool ready = !(s == 0 && k == 0);
             const bool make = !(s == nsteps - 1 && k == 4);
-            TRY(rhs_dispatch(c, c->JQ, st, nullptr, ready, make, (c->params.shock || c->keep_lag) && k == 4));
+            TRY(rhs_dispatch(c, c->JQ, st, nullptr, ready, make, (c->params.shock || c->keep_lag || c->les) && k == 4));
         }
-        if (c->params.shock || c->keep_lag) c->have_lag = true;
+        if (c->params.shock || c->keep_lag || c->les) c->have_lag = true;
     }
     if (check) {
         unsigned long long h[4];
@@ -1947,8 +2105,9 @@ int dsem_rk_step(dsem_ctx *c, double t, double dt, double *resid_out)
 static int vjp_check(dsem_ctx *c)
 {
     if (!c || !c->stream) return set_err(c, DSEM_ERR_ARG, "vjp arguments");
-    if (c->params.shock)
-        return set_err(c, DSEM_ERR_UNSUPPORTED, "the adjoint covers the path without entropy viscosity (shock = 0)");
+    if (c->params.shock || c->les)
+        return set_err(c, DSEM_ERR_UNSUPPORTED,
+                       "the adjoint covers the path without entropy viscosity or Smagorinsky model");
     if (c->params.precision != DSEM_PREC_FP64)
         return set_err(c, DSEM_ERR_UNSUPPORTED, "the adjoint is built for the FP64 path");
     HIP_TRY(c, hipSetDevice(c->device));

@@ -106,6 +106,14 @@ module dsem_hip
          type(c_ptr), value :: ctx, Ek, eps
       end function dsem_reduce_ke
 
+      ! static Smagorinsky model (input keyword 'smagorinsky', physics C_s, numax)
+      integer(c_int) function dsem_set_les(ctx, enable, cs, numax) bind(C, name='dsem_set_les')
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: enable
+         real(c_double), value :: cs, numax
+      end function dsem_set_les
+
       integer(c_int) function dsem_keep_lagged_states(ctx, enable) bind(C, name='dsem_keep_lagged_states')
          import :: c_int, c_ptr
          type(c_ptr), value :: ctx

@@ -144,6 +144,25 @@ class DSEM:
         self._check(self._L.dsem_get_lagged_states(self._h, _dp(out)))
         return out
 
+    # ------------------------------------------------------------- LES model
+    def set_les(self, enable: bool = True, cs: float = 0.17, numax: float = 1.0) -> None:
+        """Static Smagorinsky model (staticSmag.f90): nu_t = Re (C_s Delta)^2 |S|."""
+        self._check(self._L.dsem_set_les(self._h, int(enable), cs, numax))
+
+    def les_update(self):
+        """(nu_t at the Gauss points [K, np], nu_t on the staggered grids [K, 3, ns])
+        from the current lagged staggered states."""
+        g = np.zeros((self.K, self.np))
+        st = np.zeros((self.K, 3, self.ns))
+        self._check(self._L.dsem_les_update(self._h, _dp(g), _dp(st)))
+        return g, st
+
+    def set_les_viscosity(self, nut: np.ndarray) -> None:
+        nut = np.ascontiguousarray(nut, dtype=np.float64)
+        if nut.size != self.K * 3 * self.ns:
+            raise ValueError("nu_t must be [K][3][nc*ncg^2]")
+        self._check(self._L.dsem_set_les_viscosity(self._h, _dp(nut)))
+
     # ------------------------------------------------------------ diagnostics
     def keep_lagged_states(self, enable: bool = True) -> None:
         """Keep the last stage's staggered states after every step (needed by

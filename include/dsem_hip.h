@@ -194,6 +194,21 @@ int  dsem_get_state_device(dsem_ctx *ctx, double *dJQ);
  * (bmatu1b, caseFiles/tgv/BackstepProblem-stoch.f90:345-361), [ncg^3]. */
 int  dsem_quadrature_weights(const dsem_ctx *ctx, double *bmat);
 
+/* Static Smagorinsky eddy viscosity (input keyword 'smagorinsky';
+ * simpleSmag + nut_prolong, src/turbulence/staticSmago/staticSmag.f90:32-103,
+ * 206-261, called once per step by tstep, src/tstep.f90:40-43).
+ * nu_t = Re (C_s Delta)^2 |S| from the lagged staggered states, capped at
+ * numax, prolonged to the staggered grids; the viscous fluxes use
+ * amu = 1 + nu_t rho (+ mu_h), Vis_Fluxes_new.f90:123-125.  Zero on the first
+ * step after dsem_set_state.  The shock and density sensors are not built.
+ * dsem_les_update recomputes nu_t from the current lagged states and returns
+ * it at the Gauss points [K][ncg^3] and on the staggered grids
+ * [K][3][nc*ncg^2] (either nullable); dsem_set_les_viscosity prescribes the
+ * staggered field for dsem_rhs. */
+int  dsem_set_les(dsem_ctx *ctx, int enable, double cs, double numax);
+int  dsem_les_update(dsem_ctx *ctx, double *nut_g, double *nut_s);
+int  dsem_set_les_viscosity(dsem_ctx *ctx, const double *nut_s);
+
 /* Keep the last stage's face-averaged staggered states after every step
  * (d%Qlgg/Qglg/Qggl survive tstep in the reference) when entropy viscosity
  * is off, for the enstrophy diagnostic; costs one extra write of

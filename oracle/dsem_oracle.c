@@ -462,6 +462,15 @@ static int make_ops(int p, ops_t *o)
 int or_rhs(const or_mesh *mesh, const or_phys *ph, const double *JQ, const double *muh,
            double *Qt, or_debug *dbg, int nthreads)
 {
+    return or_rhs_les(mesh, ph, JQ, muh, NULL, Qt, dbg, nthreads);
+}
+
+/* or_rhs with a static-Smagorinsky eddy viscosity nut [K][3][ns] on the
+ * staggered grids: amu = 1 + nut * rho + mu_h (Vis_Fluxes_new.f90:123-125,
+ * 149-151, 177-179; rho is the face-averaged staggered density). */
+int or_rhs_les(const or_mesh *mesh, const or_phys *ph, const double *JQ, const double *muh,
+               const double *nut, double *Qt, or_debug *dbg, int nthreads)
+{
     ops_t o;
     if (make_ops(mesh->p, &o)) return -1;
 #ifdef _OPENMP
@@ -620,6 +629,7 @@ int or_rhs(const or_mesh *mesh, const or_phys *ph, const double *JQ, const doubl
                                 Gx[c][v] = s;
                             }
                         double amu = 1.0;
+                        if (nut) amu = amu + nut[((size_t)e * 3 + d) * ns + si] * Qv[0];
                         if (muh) amu = amu + muh[((size_t)e * 3 + d) * ns + si];
                         or_vis_flux(ph, Qv, Gx[0], Gx[1], Gx[2], amu, ff, gg, hh);
                         const double *gq = gmt + (size_t)si * 3;
@@ -1181,5 +1191,114 @@ int or_enstrophy(const or_mesh *mesh, const or_phys *ph, const double *JQ, const
     }
     free(part);
     *eps = 100.0 * (sum / sum2) * (1.0 / ph->re);
+    return 0;
+}
+
+
+/* ------------------------------------------------------------------ */
+/* Static Smagorinsky model                                            */
+/* ------------------------------------------------------------------ */
+/* simpleSmag (src/turbulence/staticSmago/staticSmag.f90:32-103) with
+ * filteredStrain (:126-179), delta_g (:191-201) and nut_prolong (:206-261):
+ * nu_t = Re (C_s Delta)^2 S at the Gauss points, S = (2 S_ij S_ij)^(1/2) from
+ * Gauss_Deriv of the lagged staggered states, Delta = |1/jacob|^(1/3) /
+ * (ncg^3)^(1/3), clamped to [0, numax]; prolonged to the three staggered
+ * grids with negative values set to zero.  filteredStrain takes rho =
+ * (JQ)_1 jacob at the element's first point and u = (JQ)_2 / rho, as the
+ * reference does (exact on uniform meshes).  The shock and density sensors
+ * (shockSensor, rhoSensor) are not restated.  Outputs (nullable):
+ * nut_g [K][np], nut_s [K][3][ns]. */
+int or_smagorinsky(const or_mesh *mesh, const or_phys *ph, double cs, double numax, const double *JQ,
+                   const double *stag_lag, double *nut_g, double *nut_s)
+{
+    ops_t o;
+    if (make_ops(mesh->p, &o)) return -1;
+    const int K = mesh->K, ncg = o.ncg, nc = o.nc, np = o.np, ns = o.ns;
+#pragma omp parallel for schedule(static)
+    for (int e = 0; e < K; ++e) {
+        double G[4][3][729], nu[729];
+        const double *S = stag_lag + (size_t)e * 3 * NEQ * ns;
+        const double *gmg = mesh->gmetg + (size_t)e * np * 9;
+        for (int v = 0; v < 4; ++v)
+            gauss_deriv_field(&o, gmg, S + (size_t)(0 * NEQ + v) * ns, S + (size_t)(1 * NEQ + v) * ns,
+                              S + (size_t)(2 * NEQ + v) * ns, G[v][0], G[v][1], G[v][2]);
+        const double *q = JQ + (size_t)e * NEQ * np;
+        const double *jac = mesh->jacob + (size_t)e * np;
+        const double c3 = pow((double)(ncg * ncg * ncg), 1.0 / 3.0);
+        for (int pt = 0; pt < np; ++pt) {
+            const double rho = q[0 * np + pt] * jac[0];
+            const double u = q[1 * np + pt] / rho, v = q[2 * np + pt] / rho, w = q[3 * np + pt] / rho;
+            const double rx = G[0][0][pt], ry = G[0][1][pt], rz = G[0][2][pt];
+            const double ux = (G[1][0][pt] - rx * u) / rho, uy = (G[1][1][pt] - ry * u) / rho, uz = (G[1][2][pt] - rz * u) / rho;
+            const double vx = (G[2][0][pt] - rx * v) / rho, vy = (G[2][1][pt] - ry * v) / rho, vz = (G[2][2][pt] - rz * v) / rho;
+            const double wx = (G[3][0][pt] - rx * w) / rho, wy = (G[3][1][pt] - ry * w) / rho, wz = (G[3][2][pt] - rz * w) / rho;
+            const double S11 = ux, S12 = 0.50 * (uy + vx), S13 = 0.50 * (uz + wx);
+            const double S21 = 0.50 * (vx + uy), S22 = vy, S23 = 0.50 * (vz + wy);
+            const double S31 = 0.50 * (wx + uz), S32 = 0.50 * (wy + vz), S33 = wz;
+            const double Sbar = sqrt(2 * S11 * S11 + 2 * S12 * S12 + 2 * S13 * S13 + 2 * S21 * S21 + 2 * S22 * S22
+                                     + 2 * S23 * S23 + 2 * S31 * S31 + 2 * S32 * S32 + 2 * S33 * S33);
+            const double delta = pow(fabs(1.0 / jac[pt]), 1.0 / 3.0) / c3;
+            double nt = ph->re * (cs * delta) * (cs * delta) * Sbar;
+            nu[pt] = fmin(fmax(nt, 0.0), numax);
+        }
+        if (nut_g)
+            for (int pt = 0; pt < np; ++pt) nut_g[(size_t)e * np + pt] = nu[pt];
+        if (nut_s)
+            for (int d = 0; d < 3; ++d)
+                for (int b = 0; b < ncg; ++b)
+                    for (int a = 0; a < ncg; ++a)
+                        for (int j = 0; j < nc; ++j) {
+                            double sum = 0.0;
+                            for (int k = 0; k < ncg; ++k) {
+                                double t = o.bx[j * ncg + k] * nu[gidx(d, k, a, b, ncg)];
+                                sum = (k == 0) ? t : sum + t;
+                            }
+                            if (sum < 0.0) sum = 0.0;
+                            nut_s[((size_t)e * 3 + d) * ns + sidx(d, j, a, b, ncg, nc)] = sum;
+                        }
+    }
+    return 0;
+}
+
+/* tstep with the static Smagorinsky model (src/tstep.f90:40-43): nu_t from
+ * stag_lag at the start of the step (zero when have_lag = 0, as for the
+ * entropy viscosity), five stages with amu = 1 + nu_t rho, and stag_lag
+ * replaced by the last stage's staggered states. */
+int or_rk_step_les(const or_mesh *mesh, const or_phys *ph, double cs, double numax, double *JQ, double *G,
+                   double dt, double *stag_lag, int have_lag, double *nut_s, double *resid, int nthreads)
+{
+    double ark[5], brk[5], crk[5];
+    or_rkcoefs(ark, brk, crk);
+    const int np = (mesh->p + 1) * (mesh->p + 1) * (mesh->p + 1);
+    const int ns = (mesh->p + 2) * (mesh->p + 1) * (mesh->p + 1);
+    const size_t n = (size_t)mesh->K * NEQ * np;
+    if (have_lag) {
+        int rc = or_smagorinsky(mesh, ph, cs, numax, JQ, stag_lag, NULL, nut_s);
+        if (rc) return rc;
+    } else {
+        memset(nut_s, 0, sizeof(double) * (size_t)mesh->K * 3 * ns);
+    }
+    double *Qt = (double *)malloc(sizeof(double) * n);
+    if (!Qt) return -2;
+    memset(G, 0, sizeof(double) * n);
+    double last = 0.0;
+    for (int kk = 0; kk < 5; ++kk) {
+        or_debug dbg = {kk == 4 ? stag_lag : NULL, NULL, NULL, NULL};
+        int rc = or_rhs_les(mesh, ph, JQ, NULL, nut_s, Qt, kk == 4 ? &dbg : NULL, nthreads);
+        if (rc) { free(Qt); return rc; }
+        const double a = ark[kk], c = crk[kk];
+#pragma omp parallel for schedule(static)
+        for (int e = 0; e < mesh->K; ++e)
+            for (int i = 0; i < NEQ * np; ++i) {
+                const size_t g = (size_t)e * NEQ * np + i;
+                G[g] = a * G[g] + Qt[g];
+                JQ[g] = JQ[g] + c * dt * G[g];
+            }
+        last = 0.0;
+        const double *q = Qt + (size_t)(mesh->K - 1) * NEQ * np;
+        for (int i = 0; i < NEQ * np; ++i) last = fmax(fabs(q[i]), last);
+    }
+    if (resid) *resid = last;
+    free(Qt);
     return 0;
 }

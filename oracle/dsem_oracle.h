@@ -88,6 +88,14 @@ int  or_rk_step(const or_mesh *mesh, const or_phys *ph, double *JQ, double *G, d
                 const double *muh, double *stag_out, double *resid, int nthreads);
 double or_dtcalc(const or_mesh *mesh, const or_phys *ph, const double *JQ);
 
+/* Static Smagorinsky (src/turbulence/staticSmago/staticSmag.f90, src/tstep.f90:40-43). */
+int  or_rhs_les(const or_mesh *mesh, const or_phys *ph, const double *JQ, const double *muh,
+                const double *nut, double *Qt, or_debug *dbg, int nthreads);
+int  or_smagorinsky(const or_mesh *mesh, const or_phys *ph, double cs, double numax, const double *JQ,
+                    const double *stag_lag, double *nut_g, double *nut_s);
+int  or_rk_step_les(const or_mesh *mesh, const or_phys *ph, double cs, double numax, double *JQ, double *G,
+                    double dt, double *stag_lag, int have_lag, double *nut_s, double *resid, int nthreads);
+
 /* Diagnostics (caseFiles/tgv/BackstepProblem-stoch.f90:345-361,629-692;
  * src/Energydissipation/energydissipation.f90:2-89). */
 void or_gauss_legendre(int n, double *t, double *w);

@@ -77,6 +77,12 @@ def lib():
         L.or_rk_step_ev.argtypes = [C.POINTER(OrMesh), C.POINTER(OrPhys), C.POINTER(OrEv), _dp, _dp, C.c_double,
                                     _dp, C.c_int, _dp, _dp, C.c_int]
         L.or_dtcalc.restype = C.c_double
+        L.or_rhs_les.argtypes = [C.POINTER(OrMesh), C.POINTER(OrPhys), _dp, _dp, _dp, _dp, C.POINTER(OrDebug),
+                                 C.c_int]
+        L.or_smagorinsky.argtypes = [C.POINTER(OrMesh), C.POINTER(OrPhys), C.c_double, C.c_double, _dp, _dp, _dp,
+                                     _dp]
+        L.or_rk_step_les.argtypes = [C.POINTER(OrMesh), C.POINTER(OrPhys), C.c_double, C.c_double, _dp, _dp,
+                                     C.c_double, _dp, C.c_int, _dp, _dp, C.c_int]
         L.or_gauss_legendre.argtypes = [C.c_int, _dp, _dp]
         L.or_ke_weights.argtypes = [C.c_int, _dp]
         L.or_kinetic_energy.argtypes = [C.POINTER(OrMesh), _dp, _dp]
@@ -296,3 +302,36 @@ def enstrophy(geom: Geometry, JQ, stag_lag, ph: OrPhys | None = None) -> float:
     assert lib().or_enstrophy(C.byref(m), C.byref(ph), ptr(np.ascontiguousarray(JQ)),
                               ptr(np.ascontiguousarray(stag_lag)), C.byref(out)) == 0
     return out.value
+
+
+# ---------------------------------------------------------------- static Smagorinsky
+def smagorinsky(geom: Geometry, JQ, stag_lag, cs: float, numax: float, ph: OrPhys | None = None):
+    """(nu_t at the Gauss points [K,np], nu_t on the staggered grids [K,3,ns])."""
+    ph = ph or phys()
+    g = np.zeros((geom.K, geom.np))
+    st = np.zeros((geom.K, 3, geom.ns))
+    m = geom.cmesh()
+    assert lib().or_smagorinsky(C.byref(m), C.byref(ph), cs, numax, ptr(np.ascontiguousarray(JQ)),
+                                ptr(np.ascontiguousarray(stag_lag)), ptr(g), ptr(st)) == 0
+    return g, st
+
+
+def rhs_les(geom: Geometry, JQ, nut, ph: OrPhys | None = None, nthreads=0):
+    ph = ph or phys()
+    Qt = np.zeros_like(JQ)
+    m = geom.cmesh()
+    assert lib().or_rhs_les(C.byref(m), C.byref(ph), ptr(np.ascontiguousarray(JQ)), None,
+                            ptr(np.ascontiguousarray(nut, dtype=np.float64)), ptr(Qt), None, nthreads) == 0
+    return Qt
+
+
+def rk_step_les(geom: Geometry, JQ, dt, stag_lag, have_lag, cs, numax, ph: OrPhys | None = None, nthreads=0):
+    """One static-Smagorinsky step in place; stag_lag updated in place.  Returns (JQ, nut_s, resid)."""
+    ph = ph or phys()
+    G = np.zeros_like(JQ)
+    nut = np.zeros((geom.K, 3, geom.ns))
+    res = C.c_double(0.0)
+    m = geom.cmesh()
+    assert lib().or_rk_step_les(C.byref(m), C.byref(ph), cs, numax, ptr(JQ), ptr(G), dt, ptr(stag_lag),
+                                int(have_lag), ptr(nut), C.byref(res), nthreads) == 0
+    return JQ, nut, res.value

@@ -20,7 +20,7 @@ def test_library_exports_every_declared_symbol():
 
 def test_timing_table_is_static():
     lib = L.load()
-    assert lib.dsem_timing_count() == 9
+    assert lib.dsem_timing_count() == 10
 
 
 def _create(mesh, **kw):
