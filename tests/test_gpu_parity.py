@@ -34,8 +34,9 @@ def _case(mesh, variant="isothermal", perturb=1e-3, seed=1234):
     return g, JQ
 
 
-@pytest.mark.parametrize("n,p,metrics", [(2, 3, "per_point"), (3, 4, "per_element"), (3, 4, "per_point"),
-                                         (2, 8, "auto"), (4, 6, "auto")])
+@pytest.mark.parametrize("n,p,metrics", [(3, 1, "per_point"), (3, 2, "per_point"), (2, 3, "per_point"),
+                                         (3, 4, "per_element"), (3, 4, "per_point"), (2, 5, "per_point"),
+                                         (4, 6, "auto"), (2, 7, "per_point"), (2, 8, "auto")])
 def test_rhs_matches_oracle(n, p, metrics):
     print(f"\n[case] rhs n={n} p={p} metrics={metrics}", flush=True)
     mesh = M.periodic_box(n, p)
@@ -173,6 +174,22 @@ def test_full_size_rhs_matches_oracle(metrics):
         per_element_metrics(g)
     want = O.rhs(g, JQ, O.phys(re=1600.0))
     with DSEM(mesh, re=1600.0, metrics=metrics) as s:
+        s.set_state(JQ)
+        got = s.rhs()
+    err = rel_l2(got, want)
+    print("rel L2 per field", err, flush=True)
+    assert np.all(err <= TOL), err
+
+
+@pytest.mark.parametrize("p", [1, 2, 5, 7])
+def test_rhs_distorted_mesh_every_order(p):
+    """Non-affine elements at the orders not covered above."""
+    print(f"\n[case] rhs distorted 2^3 p={p}", flush=True)
+    mesh = distorted_box(2, p, amp=0.07, seed=10 + p)
+    g, JQ = _case(mesh, perturb=1e-2, seed=p)
+    ph = O.phys(re=300.0)
+    want = O.rhs(g, JQ, ph)
+    with DSEM(mesh, re=300.0, metrics="per_point") as s:
         s.set_state(JQ)
         got = s.rhs()
     err = rel_l2(got, want)
