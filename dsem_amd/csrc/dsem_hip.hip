@@ -50,7 +50,12 @@ namespace {
 
 constexpr int NEQ = 5;
 constexpr int BLOCK = 256;
-constexpr int VBLOCK = 512;   // fused volume kernel: one element per workgroup, 8 waves
+// Fused volume kernel: one element per workgroup.  FP64 holds 149 KB of LDS
+// per element, so one workgroup fills a CU; 1024 work-items (16 waves, 127
+// VGPRs) keep 4 waves per SIMD in flight.  The FP32 modes (75 KB) fit two
+// workgroups of 512 per CU.
+template <int PM>
+constexpr int vblock() { return PM == 0 ? 1024 : 512; }
 
 // operators for every order p = 1..8, indexed [p]
 __constant__ double c_D[9][90];   // D[k*nc + j]
@@ -350,12 +355,13 @@ __device__ __forceinline__ void volume_grid(const DevMesh &m, const DevPhys &ph,
                                             const typename Ctr<PM>::T *sQ,
                                             const typename Ctr<PM>::T *sG, typename Ctr<PM>::T *sF,
                                             const double *sBar, int e, const int *mort,
-                                            double (&acc)[(Sz<P>::NP + VBLOCK - 1) / VBLOCK][5])
+                                            double (&acc)[(Sz<P>::NP + vblock<PM>() - 1) / vblock<PM>()][5])
 {
     using S = Sz<P>;
     using CT = typename Ctr<PM>::T;
     using CA = typename Ctr<PM>::A;
     static_assert(S::NC % NPT == 0, "paired points need an even nc");
+    constexpr int VBLOCK = vblock<PM>();
     constexpr int R = (S::NP + VBLOCK - 1) / VBLOCK;
     constexpr int JH = S::NC / NPT;    // j distance between the points of a work-item
     constexpr int NW = S::NS / NPT;    // work-items per grid
@@ -543,7 +549,7 @@ __device__ __forceinline__ void volume_grid(const DevMesh &m, const DevPhys &ph,
 // Pointwise physics, the gradient metric combination inputs, the RK state
 // and all face kernels stay FP64.
 template <int P, int PM>
-__global__ __launch_bounds__(VBLOCK) void k_volume(const double *__restrict__ JQ, DevMesh m, DevPhys ph,
+__global__ __launch_bounds__(vblock<PM>()) void k_volume(const double *__restrict__ JQ, DevMesh m, DevPhys ph,
                                                     const double *__restrict__ QA, const double *__restrict__ muh,
                                                     const double *__restrict__ nut, double *__restrict__ FV,
                                                     double *__restrict__ Qtp, double *__restrict__ Slag)
@@ -551,6 +557,7 @@ __global__ __launch_bounds__(VBLOCK) void k_volume(const double *__restrict__ JQ
     using S = Sz<P>;
     using CT = typename Ctr<PM>::T;
     using CA = typename Ctr<PM>::A;
+    constexpr int VBLOCK = vblock<PM>();
     constexpr int R = (S::NP + VBLOCK - 1) / VBLOCK;
     __shared__ CT sQ[NEQ * S::NP];            // physical Q at Gauss points (minus Qbar when mixed)
     __shared__ CT sG[15 * S::NP];             // dQ_v/dx_c at Gauss points, (c*5+v)
@@ -641,10 +648,9 @@ __global__ __launch_bounds__(VBLOCK) void k_volume(const double *__restrict__ JQ
     for (int r = 0; r < R; ++r)
 #pragma unroll
         for (int v = 0; v < 5; ++v) acc[r][v] = 0.0;
-    // FP64: the element's LDS allows one workgroup per CU whatever the VGPR
-    // count, so two points per work-item cost no occupancy; the FP32 modes
-    // fit two workgroups per CU and keep one point per work-item.
-    constexpr int NPT = (PM == 0 && Sz<P>::NC % 2 == 0) ? 2 : 1;
+    // one staggered point per work-item (two per work-item, sharing the line
+    // loads, measured slower once the FP64 workgroup has 1024 work-items)
+    constexpr int NPT = 1;
     volume_grid<P, 0, PM, NPT>(m, ph, QA, muh, nut, FV, Slag, sQ, sG, sF, sBar, e, mort, acc);
     volume_grid<P, 1, PM, NPT>(m, ph, QA, muh, nut, FV, Slag, sQ, sG, sF, sBar, e, mort, acc);
     volume_grid<P, 2, PM, NPT>(m, ph, QA, muh, nut, FV, Slag, sQ, sG, sF, sBar, e, mort, acc);
@@ -1217,6 +1223,11 @@ __global__ __launch_bounds__(BLOCK) void k_dtcalc(const double *__restrict__ JQ,
 // ====================================================================== host
 namespace {
 
+// Carpenter-Kennedy 2N RK4(5) coefficients as rkcoefs(4) writes them
+// (src/Independentroutines/baselib.f90:135-152): ark = A_k, crk = B_k.
+const double kArk[5] = {0.0, -0.4178904745, -1.192151694643, -1.697784692471, -1.514183444257};
+const double kCrk[5] = {0.1496590219993, 0.3792103129999, 0.8229550293869, 0.6994504559488, 0.1530572479681};
+
 enum KernelId { KID_TRACES = 0, KID_FACE_INV, KID_VOLUME, KID_FACE_VISC, KID_UPDATE, KID_DTCALC, KID_EV, KID_DIAG, KID_ADJ, KID_LES, KID_COUNT };
 const char *const kKernelNames[KID_COUNT] = {"k_traces", "k_face_inv", "k_volume", "k_face_visc",
                                              "k_update", "k_dtcalc", "k_ev", "k_diag", "k_adj", "k_smag"};
@@ -1376,11 +1387,11 @@ int run_rhs(dsem_ctx *c, double *JQ, const StageArgs &st, double *Qt_out, bool t
     double *slag = save_lag ? c->Slag : nullptr;
     TRY(launch(c, KID_VOLUME, [&] {
         if (c->params.precision == DSEM_PREC_FP32)
-            k_volume<P, 1><<<K, VBLOCK, 0, s>>>(JQ, m, ph, c->QA, muh, nut, c->FV, c->Qtp, slag);
+            k_volume<P, 1><<<K, vblock<1>(), 0, s>>>(JQ, m, ph, c->QA, muh, nut, c->FV, c->Qtp, slag);
         else if (c->params.precision == DSEM_PREC_BF16X3)
-            k_volume<P, 2><<<K, VBLOCK, 0, s>>>(JQ, m, ph, c->QA, muh, nut, c->FV, c->Qtp, slag);
+            k_volume<P, 2><<<K, vblock<2>(), 0, s>>>(JQ, m, ph, c->QA, muh, nut, c->FV, c->Qtp, slag);
         else
-            k_volume<P, 0><<<K, VBLOCK, 0, s>>>(JQ, m, ph, c->QA, muh, nut, c->FV, c->Qtp, slag);
+            k_volume<P, 0><<<K, vblock<0>(), 0, s>>>(JQ, m, ph, c->QA, muh, nut, c->FV, c->Qtp, slag);
     }));
     TRY(launch(c, KID_FACE_VISC, [&] { k_face_visc<P><<<face_blocks, BLOCK, 0, s>>>(c->FV, c->FI, m, ph, c->FT); }));
     if (make_traces)
@@ -1603,8 +1614,6 @@ int tape_reserve(dsem_ctx *c, int nslots)
     return DSEM_OK;
 }
 
-static const double kArk[5] = {0.0, -0.4178904745, -1.192151694643, -1.697784692471, -1.514183444257};
-static const double kCrk[5] = {0.1496590219993, 0.3792103129999, 0.8229550293869, 0.6994504559488, 0.1530572479681};
 
 // One RK step of c->JQ (advanced in place) that records each stage's input
 // state and forward intermediates into tape slot `slot`.
@@ -2060,8 +2069,6 @@ int dsem_rk_steps(dsem_ctx *c, double t, double dt, int nsteps, double *resid_ou
 {
     (void)t;
     if (!c || !c->stream || nsteps < 1) return set_err(c, DSEM_ERR_ARG, "rk_steps arguments");
-    static const double ark[5] = {0.0, -0.4178904745, -1.192151694643, -1.697784692471, -1.514183444257};
-    static const double crk[5] = {0.1496590219993, 0.3792103129999, 0.8229550293869, 0.6994504559488, 0.1530572479681};
     HIP_TRY(c, hipSetDevice(c->device));
     const bool check = resid_out != nullptr;
     if (check) HIP_TRY(c, hipMemsetAsync(c->d_scalars, 0, sizeof(unsigned long long) * 4, c->stream));
@@ -2070,8 +2077,8 @@ int dsem_rk_steps(dsem_ctx *c, double t, double dt, int nsteps, double *resid_ou
         TRY(les_step(c, false));
         for (int k = 0; k < 5; ++k) {
             StageArgs st;
-            st.a = ark[k];
-            st.cdt = crk[k] * dt;
+            st.a = kArk[k];
+            st.cdt = kCrk[k] * dt;
             st.first = (k == 0);
             st.rk = 1;
             const bool last = check && s == nsteps - 1 && k == 4;
