@@ -77,7 +77,7 @@ def run(n=8, p=6, steps=50, every=10, ndir=32, h=1e-6, seed=2024, re=1600.0, cfl
         t2 = time.perf_counter()
         gvel = vel.grad.detach().cpu().numpy()
         gnu = float(nu.grad)
-        out["J"] = float(J)
+        out["J"] = float(J.detach())
         out["forward_ms"] = (t1 - t0) * 1e3
         out["backward_ms"] = (t2 - t1) * 1e3
         out["adjoint_over_forward"] = (t2 - t1) / (t1 - t0)

@@ -76,6 +76,46 @@ def test_rhs_vjp_field_by_field():
             assert _rel(ad, fd) < 1e-6, (f, ad, fd)
 
 
+def test_rhs_vjp_transonic_states():
+    """Sonic flow: u close to +c and v close to -c, so that on x faces the
+    Roe flux takes the qtd >= 0 branch and on y faces the qtd < 0 branch,
+    each with its eigenvalue (qtd -+ atd) near zero, where the entropy fix
+    switches on.  TGV states at Mach 0.1 never reach that fix."""
+    print("\n[case] rhs vjp transonic 3^3 p=3", flush=True)
+    mesh = distorted_box(3, 3, amp=0.04, seed=6)
+    g = O.Geometry(mesh)
+    gamma = 1.4
+    rng = np.random.default_rng(21)
+    shp = g.jacob.shape
+    rho = 1.0 + 0.1 * rng.standard_normal(shp)
+    p = 1.0 / gamma * (1.0 + 0.1 * rng.standard_normal(shp))
+    c = np.sqrt(gamma * p / rho)
+    u = 1.0 + 0.15 * rng.standard_normal(shp)
+    v = -1.0 + 0.15 * rng.standard_normal(shp)
+    w = 0.15 * rng.standard_normal(shp)
+    # a sizeable share of points lies within 5% of sonic along each direction
+    assert np.mean(np.abs(u / c - 1.0) < 0.05) > 0.1
+    assert np.mean(np.abs(v / c + 1.0) < 0.05) > 0.1
+    Q = np.ascontiguousarray(np.transpose(cases.conservative(rho, u, v, w, p, gamma), (1, 0, 2)))
+    JQ = Q / g.jacob[:, None, :]
+    with DSEM(mesh, re=200.0, metrics="per_point") as s:
+        s.set_state(JQ)
+        for trial in range(3):
+            lam = rng.standard_normal(JQ.shape)
+            dv = rng.standard_normal(JQ.shape) * np.abs(JQ).mean(axis=(0, 2), keepdims=True)
+            jb, _ = s.rhs_vjp(lam)
+            h = 1e-7
+            s.set_state(JQ + h * dv)
+            rp = s.rhs()
+            s.set_state(JQ - h * dv)
+            rm = s.rhs()
+            s.set_state(JQ)
+            fd = float(np.sum(lam * (rp - rm)) / (2 * h))
+            ad = float(np.sum(jb * dv))
+            print(f"trial {trial}: adjoint {ad:.12e} fd {fd:.12e} rel {_rel(ad, fd):.2e}", flush=True)
+            assert _rel(ad, fd) < 1e-6
+
+
 def test_nu_adjoint_matches_difference_in_reynolds_number():
     print("\n[case] nu adjoint 3^3 p=4", flush=True)
     mesh = M.periodic_box(3, 4)
