@@ -341,169 +341,261 @@ __device__ __forceinline__ void load_physical_state(const double *__restrict__ J
     for (int i = threadIdx.x; i < NEQ * S::NP; i += BLOCK) sQ[i] = q[i] * jac[(i % S::NP) * m.ms];
 }
 
+// Line contractions of the FP64 staggered pass on the matrix cores.
+// v_mfma_f64_4x4x4f64 runs four independent 4x4x4 products per wave.  Lane l
+// (c = l%4, blk = (l/4)%4, r = l/16) supplies A_blk[c][r] and B_blk[r][c]
+// and receives D_blk[r][c] (layout measured on gfx950).  With A = four rows
+// of the Gauss->Lobatto interpolation matrix B and B = four Gauss values of
+// four lines, D[r][c] is the value at Lobatto index 4 jt + r of line 4 g + c,
+// the point this lane owns: the pointwise pipeline that follows runs on it
+// unchanged, and each Gauss value is read from LDS once per 4x4 tile instead
+// of once per output point.  A grid of nc x ncg^2 points is 4x4 tiles over
+// (Lobatto index, line); tile t = 4 * wave + blk, t = g * JT + jt.
+template <int P>
+struct MfTiles {
+    using S = Sz<P>;
+    static constexpr int JT = (S::NC + 3) / 4;     // tiles along the Lobatto index
+    static constexpr int KS = (S::NCG + 3) / 4;    // steps of 4 Gauss points
+    static constexpr int NL = S::NCG * S::NCG;     // lines per grid
+    static constexpr int NT = ((NL + 3) / 4) * JT; // tiles per grid
+};
+
+template <int P>
+struct MfLane {
+    double A[MfTiles<P>::KS];   // B[4 jt + c][4 ks + r], zero outside B
+    int r, j, line;             // line is also the line this lane loads
+    bool lvalid;                // tile and line exist (the lane loads)
+    bool valid;                 // and its Lobatto index exists (the lane owns a point)
+    bool wave_active;           // some tile of this wave exists (uniform per wave)
+};
+
+template <int P>
+__device__ __forceinline__ MfLane<P> mf_lane(int tid)
+{
+    using S = Sz<P>;
+    using T = MfTiles<P>;
+    MfLane<P> ml;
+    const int l = tid % 64, c = l % 4, blk = (l / 4) % 4;
+    ml.r = l / 16;
+    const int t = (tid / 64) * 4 + blk;
+    const int jt = t % T::JT, g = t / T::JT;
+    ml.j = 4 * jt + ml.r;
+    ml.line = 4 * g + c;
+    ml.lvalid = t < T::NT && ml.line < T::NL;
+    ml.valid = ml.lvalid && ml.j < S::NC;
+    ml.wave_active = (tid / 64) * 4 < T::NT;
+    if (!ml.lvalid) ml.line = 0;
+#pragma unroll
+    for (int ks = 0; ks < T::KS; ++ks) {
+        const int row = 4 * jt + c, col = 4 * ks + ml.r;
+        ml.A[ks] = (row < S::NC && col < S::NCG) ? c_B[P][row * S::NCG + col] : 0.0;
+    }
+    return ml;
+}
+
+// sum_k B[j][k] fld[k along the lane's line], for the point the lane owns;
+// off/ok: LDS offsets and validity of the Gauss values this lane supplies.
+template <int P>
+__device__ __forceinline__ double mf_line(const double *fld, const MfLane<P> &ml, const int *off, const bool *ok)
+{
+    double t = 0.0;
+#pragma unroll
+    for (int ks = 0; ks < MfTiles<P>::KS; ++ks) {
+        const double bv = ok[ks] ? fld[off[ks]] : 0.0;
+        t = __builtin_amdgcn_mfma_f64_4x4x4f64(ml.A[ks], bv, t, 0, 0, 0);
+    }
+    return t;
+}
+
 // One staggered grid D of the fused volume kernel: fluxes at the nc*ncg^2
 // points of grid D into LDS (interior) or the viscous face traces (faces),
 // then the interior part of the divergence along D added into acc.
-// NPT staggered points per work-item: with NPT = 2 a work-item takes the
-// points j and j + nc/2 of one line, so the 9 LDS values of the line are
-// loaded once for both (interpolation of Q and prolongation of the 15
-// gradients); the arithmetic of each point is unchanged.
-template <int P, int D, int PM, int NPT>
+// MF (FP64 only): the interpolation of Q and the prolongation of the 15
+// gradients run on the matrix cores (mf_line), one point per lane; otherwise
+// each work-item forms its point's 9-term sums on the vector ALUs.
+template <int P, int D, int PM, bool MF>
 __device__ __forceinline__ void volume_grid(const DevMesh &m, const DevPhys &ph, const double *__restrict__ QA,
                                             const double *__restrict__ muh, const double *__restrict__ nut,
                                             double *__restrict__ FV, double *__restrict__ Slag,
                                             const typename Ctr<PM>::T *sQ,
                                             const typename Ctr<PM>::T *sG, typename Ctr<PM>::T *sF,
-                                            const double *sBar, int e, const int *mort,
+                                            const double *sBar, const typename Ctr<PM>::T *sD,
+                                            const typename Ctr<PM>::T *sB, const double *sDint, int e,
+                                            const int *mort, const MfLane<P> &ml,
                                             double (&acc)[(Sz<P>::NP + vblock<PM>() - 1) / vblock<PM>()][5])
 {
     using S = Sz<P>;
     using CT = typename Ctr<PM>::T;
     using CA = typename Ctr<PM>::A;
-    static_assert(S::NC % NPT == 0, "paired points need an even nc");
+    static_assert(!MF || PM == 0, "matrix-core contractions are FP64 only");
     constexpr int VBLOCK = vblock<PM>();
+    static_assert(!MF || MfTiles<P>::NT <= VBLOCK / 16, "one tile per 16 lanes in one pass");
     constexpr int R = (S::NP + VBLOCK - 1) / VBLOCK;
-    constexpr int JH = S::NC / NPT;    // j distance between the points of a work-item
-    constexpr int NW = S::NS / NPT;    // work-items per grid
+    constexpr int KS = MfTiles<P>::KS;
     const int tid = threadIdx.x;
     const int nsm = m.ms ? S::NS : 1, nfm = m.ms ? S::NF : 1;
     const double *gmd = (D == 0 ? m.gmet0 : (D == 1 ? m.gmet1 : m.gmet2)) + (size_t)e * 3 * nsm;
-    __syncthreads();   // sG complete / previous divergence pass done with sF
-#pragma unroll 1
-    for (int w = tid; w < NW; w += VBLOCK) {
-        int a, b, j[NPT];
-        if constexpr (NPT == 1) {
-            const int s0 = w;
-            if (D == 0) { j[0] = s0 % S::NC; a = (s0 / S::NC) % S::NCG; b = s0 / (S::NC * S::NCG); }
-            else if (D == 1) { a = s0 % S::NCG; j[0] = (s0 / S::NCG) % S::NC; b = s0 / (S::NCG * S::NC); }
-            else { a = s0 % S::NCG; b = (s0 / S::NCG) % S::NCG; j[0] = s0 / (S::NCG * S::NCG); }
-        } else {
-            const int line = w / JH, jj = w % JH;
-            a = line % S::NCG;
-            b = line / S::NCG;
+    // Matrix-core form: the lane's point is fixed, so its metric row and face
+    // values are loaded before the barrier and arrive while the other waves
+    // finish the previous pass.
+    int mf_s = 0;
+    double mf_g[3] = {0.0, 0.0, 0.0}, mf_qa[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    double mf_nrm = 1.0, mf_nut = 0.0, mf_muh = 0.0;
+    if constexpr (MF) {
+        const int a = ml.line % S::NCG, b = ml.line / S::NCG;
+        mf_s = sidx<P>(D, ml.valid ? ml.j : 0, a, b);
+        if (ml.valid) {
 #pragma unroll
-            for (int u = 0; u < NPT; ++u) j[u] = jj + u * JH;
+            for (int i = 0; i < 3; ++i) mf_g[i] = gmd[i * nsm + mf_s * m.ms];
+            if (nut) mf_nut = nut[((size_t)e * 3 + D) * S::NS + mf_s];
+            if (muh) mf_muh = muh[((size_t)e * 3 + D) * S::NS + mf_s];
+            if (ml.j == 0 || ml.j == S::NC - 1) {
+                const int mo = mort[face_of<P>(D, ml.j) - 1];
+                const int q = a + S::NCG * b;
+#pragma unroll
+                for (int v = 0; v < 5; ++v) mf_qa[v] = QA[((size_t)mo * NEQ + v) * S::NF + q];
+                mf_nrm = m.norm[(size_t)mo * nfm + q * m.ms];
+            }
+        }
+    }
+    __syncthreads();   // sG complete / previous divergence pass done with sF
+    // Matrix-core form: one pass, every lane of an active wave takes part in
+    // the contractions (the tiles need all 64 lanes); only lanes that own a
+    // point store.  Vector form: work-items stride over the points.
+    const int w0 = MF ? 0 : tid;
+    const int wend = MF ? (ml.wave_active ? 1 : 0) : S::NS;
+    const int wstep = MF ? 1 : VBLOCK;
+#pragma unroll 1
+    for (int w = w0; w < wend; w += wstep) {
+        int a, b, j;
+        bool valid = true;
+        int off[KS];
+        bool ok[KS];
+        if constexpr (MF) {
+            j = ml.j;
+            a = ml.line % S::NCG;
+            b = ml.line / S::NCG;
+            valid = ml.valid;
+#pragma unroll
+            for (int ks = 0; ks < KS; ++ks) {
+                const int k = 4 * ks + ml.r;
+                ok[ks] = ml.lvalid && k < S::NCG;
+                off[ks] = ok[ks] ? gidx<P>(D, k, a, b) : 0;
+            }
+        } else {
+            const int s0 = w;
+            if (D == 0) { j = s0 % S::NC; a = (s0 / S::NC) % S::NCG; b = s0 / (S::NC * S::NCG); }
+            else if (D == 1) { a = s0 % S::NCG; j = (s0 / S::NCG) % S::NC; b = s0 / (S::NCG * S::NC); }
+            else { a = s0 % S::NCG; b = (s0 / S::NCG) % S::NCG; j = s0 / (S::NCG * S::NCG); }
         }
         const int q = a + S::NCG * b;
-        int s[NPT], fc[NPT], mo[NPT];
-        bool face[NPT];
-        CT Bj[NPT][S::NCG];
+        const int jc = valid ? j : 0;
+        const int s = MF ? mf_s : sidx<P>(D, jc, a, b);
+        const bool face = (jc == 0 || jc == S::NC - 1);
+        const int fc = face_of<P>(D, jc);
+        const int mo = mort[fc - 1];
+        double Qv[5];
+        CT Bj[MF ? 1 : S::NCG];
+        if constexpr (MF) {
 #pragma unroll
-        for (int u = 0; u < NPT; ++u) {
-            s[u] = sidx<P>(D, j[u], a, b);
-            face[u] = (j[u] == 0 || j[u] == S::NC - 1);
-            fc[u] = face_of<P>(D, j[u]);
-            mo[u] = mort[fc[u] - 1];
-#pragma unroll
-            for (int k = 0; k < S::NCG; ++k) Bj[u][k] = opB<PM>(P, j[u] * S::NCG + k);
-        }
-        double Qv[NPT][5];
-#pragma unroll
-        for (int v = 0; v < 5; ++v) {
-            CT ql[S::NCG];
-#pragma unroll
-            for (int k = 0; k < S::NCG; ++k) ql[k] = sQ[v * S::NP + gidx<P>(D, k, a, b)];
-#pragma unroll
-            for (int u = 0; u < NPT; ++u) {
-                if (face[u]) {
-                    Qv[u][v] = QA[((size_t)mo[u] * NEQ + v) * S::NF + q];
-                } else {
-                    CA t = 0;
-#pragma unroll
-                    for (int k = 0; k < S::NCG; ++k) t = mac<PM>(Bj[u][k], ql[k], t);
-                    if constexpr (PM == 0) Qv[u][v] = t; else Qv[u][v] = sBar[v] + (double)t;
-                }
+            for (int v = 0; v < 5; ++v) {
+                const double t = mf_line<P>(sQ + v * S::NP, ml, off, ok);
+                Qv[v] = face ? mf_qa[v] : t;
             }
-            if (v == 1 || v == 3) __builtin_amdgcn_sched_barrier(0);
+        } else {
+#pragma unroll
+            for (int k = 0; k < S::NCG; ++k) Bj[k] = sB[j * S::NCG + k];
+#pragma unroll
+            for (int v = 0; v < 5; ++v) {
+                if (face) {
+                    Qv[v] = QA[((size_t)mo * NEQ + v) * S::NF + q];
+                } else {
+                    CA u = 0;
+#pragma unroll
+                    for (int k = 0; k < S::NCG; ++k) u = mac<PM>(Bj[k], sQ[v * S::NP + gidx<P>(D, k, a, b)], u);
+                    if constexpr (PM == 0) Qv[v] = u; else Qv[v] = sBar[v] + (double)u;
+                }
+                if (v == 1 || v == 3) __builtin_amdgcn_sched_barrier(0);
+            }
         }
-        if (Slag) {
+        if (Slag && valid) {
 #pragma unroll
-            for (int u = 0; u < NPT; ++u)
-#pragma unroll
-                for (int v = 0; v < 5; ++v) Slag[(((size_t)e * 3 + D) * NEQ + v) * S::NS + s[u]] = Qv[u][v];
+            for (int v = 0; v < 5; ++v) Slag[(((size_t)e * 3 + D) * NEQ + v) * S::NS + s] = Qv[v];
         }
-        Prim pr[NPT];
-#pragma unroll
-        for (int u = 0; u < NPT; ++u) pr[u] = primitives(ph, Qv[u]);
+        if constexpr (MF) {
+            if (!valid) { Qv[0] = 1.0; Qv[1] = Qv[2] = Qv[3] = 0.0; Qv[4] = 1.0; }   // no point: keep the arithmetic finite
+        }
+        const Prim pr = primitives(ph, Qv);
         // Velocity and temperature gradients, one direction c at a time: the
         // prolonged conservative gradient dQ/dx_c (5 fields) is turned into
         // (du/dx_c, dv/dx_c, dw/dx_c, dT/dx_c) and folded into the symmetric
         // velocity-gradient sums, so no more than one direction's loads are live.
-        double Sxx[NPT], Syy[NPT], Szz[NPT], Sxy[NPT], Sxz[NPT], Syz[NPT], div[NPT], Tx[NPT], Ty[NPT], Tz[NPT];
-#pragma unroll
-        for (int u = 0; u < NPT; ++u) {
-            Sxx[u] = Syy[u] = Szz[u] = Sxy[u] = Sxz[u] = Syz[u] = div[u] = 0.0;
-            Tx[u] = Ty[u] = Tz[u] = 0.0;
-        }
+        double Sxx = 0.0, Syy = 0.0, Szz = 0.0, Sxy = 0.0, Sxz = 0.0, Syz = 0.0, div = 0.0;
+        double Tx = 0.0, Ty = 0.0, Tz = 0.0;
 #pragma unroll 1
         for (int c = 0; c < 3; ++c) {
-            double Gc[NPT][5];
+            double Gc[5];
 #pragma unroll
             for (int v = 0; v < 5; ++v) {
                 const CT *gl = sG + (c * 5 + v) * S::NP;
-                CT gv[S::NCG];
+                if constexpr (MF) {
+                    Gc[v] = mf_line<P>(gl, ml, off, ok);
+                } else {
+                    CA u = 0;
 #pragma unroll
-                for (int k = 0; k < S::NCG; ++k) gv[k] = gl[gidx<P>(D, k, a, b)];
-#pragma unroll
-                for (int u = 0; u < NPT; ++u) {
-                    CA t = 0;
-#pragma unroll
-                    for (int k = 0; k < S::NCG; ++k) t = mac<PM>(Bj[u][k], gv[k], t);
-                    Gc[u][v] = (double)t;
+                    for (int k = 0; k < S::NCG; ++k) u = mac<PM>(Bj[k], gl[gidx<P>(D, k, a, b)], u);
+                    Gc[v] = (double)u;
                 }
             }
-#pragma unroll
-            for (int u = 0; u < NPT; ++u) {
-                double o[4];
-                prim_gradient(ph, pr[u], Gc[u], o);
-                const double uc = o[0], vc = o[1], wc = o[2];
-                div[u] = div[u] + (c == 0 ? uc : (c == 1 ? vc : wc));
-                Sxx[u] += (c == 0) ? uc : 0.0;
-                Syy[u] += (c == 1) ? vc : 0.0;
-                Szz[u] += (c == 2) ? wc : 0.0;
-                Sxy[u] += (c == 0) ? vc : ((c == 1) ? uc : 0.0);   // uy + vx
-                Sxz[u] += (c == 0) ? wc : ((c == 2) ? uc : 0.0);   // wx + uz
-                Syz[u] += (c == 1) ? wc : ((c == 2) ? vc : 0.0);   // wy + vz
-                Tx[u] = (c == 0) ? o[3] : Tx[u];
-                Ty[u] = (c == 1) ? o[3] : Ty[u];
-                Tz[u] = (c == 2) ? o[3] : Tz[u];
-            }
+            double o[4];
+            prim_gradient(ph, pr, Gc, o);
+            const double uc = o[0], vc = o[1], wc = o[2];
+            div = div + (c == 0 ? uc : (c == 1 ? vc : wc));
+            Sxx += (c == 0) ? uc : 0.0;
+            Syy += (c == 1) ? vc : 0.0;
+            Szz += (c == 2) ? wc : 0.0;
+            Sxy += (c == 0) ? vc : ((c == 1) ? uc : 0.0);   // uy + vx
+            Sxz += (c == 0) ? wc : ((c == 2) ? uc : 0.0);   // wx + uz
+            Syz += (c == 1) ? wc : ((c == 2) ? vc : 0.0);   // wy + vz
+            Tx = (c == 0) ? o[3] : Tx;
+            Ty = (c == 1) ? o[3] : Ty;
+            Tz = (c == 2) ? o[3] : Tz;
         }
-#pragma unroll
-        for (int u = 0; u < NPT; ++u) {
+        if (valid) {
             double amu = 1.0;
-            if (nut) amu = amu + nut[((size_t)e * 3 + D) * S::NS + s[u]] * Qv[u][0];   // static Smagorinsky
-            if (muh) amu = amu + muh[((size_t)e * 3 + D) * S::NS + s[u]];
-            const int sq = s[u] * m.ms;
-            const double g0 = gmd[0 * nsm + sq], g1 = gmd[1 * nsm + sq], g2 = gmd[2 * nsm + sq];
+            if (nut) amu = amu + (MF ? mf_nut : nut[((size_t)e * 3 + D) * S::NS + s]) * Qv[0];   // static Smagorinsky
+            if (muh) amu = amu + (MF ? mf_muh : muh[((size_t)e * 3 + D) * S::NS + s]);
+            const int sq = s * m.ms;
+            const double g0 = MF ? mf_g[0] : gmd[0 * nsm + sq];
+            const double g1 = MF ? mf_g[1] : gmd[1 * nsm + sq];
+            const double g2 = MF ? mf_g[2] : gmd[2 * nsm + sq];
             double rv[5];
             {
                 // Stokes stress and Fourier heat flux (x/y/z_vis_flux, PhysLib.f90:464-470)
                 const double hfac = (1.0 + (amu - 1.0) * ph.kap) * ph.hfac1;
-                const double d3 = div[u] * (1.0 / 3.0);
-                const double txx = 2.0 * amu * (Sxx[u] - d3), tyy = 2.0 * amu * (Syy[u] - d3), tzz = 2.0 * amu * (Szz[u] - d3);
-                const double txy = amu * Sxy[u], txz = amu * Sxz[u], tyz = amu * Syz[u];
-                const Prim &p_ = pr[u];
-                const double f4 = p_.u * txx + p_.v * txy + p_.w * txz + hfac * Tx[u];
-                const double g4 = p_.v * tyy + p_.u * txy + p_.w * tyz + hfac * Ty[u];
-                const double h4 = p_.v * tyz + p_.u * txz + p_.w * tzz + hfac * Tz[u];
+                const double d3 = div * (1.0 / 3.0);
+                const double txx = 2.0 * amu * (Sxx - d3), tyy = 2.0 * amu * (Syy - d3), tzz = 2.0 * amu * (Szz - d3);
+                const double txy = amu * Sxy, txz = amu * Sxz, tyz = amu * Syz;
+                const double f4 = pr.u * txx + pr.v * txy + pr.w * txz + hfac * Tx;
+                const double g4 = pr.v * tyy + pr.u * txy + pr.w * tyz + hfac * Ty;
+                const double h4 = pr.v * tyz + pr.u * txz + pr.w * tzz + hfac * Tz;
                 rv[0] = 0.0;
                 rv[1] = g0 * txx + g1 * txy + g2 * txz;
                 rv[2] = g0 * txy + g1 * tyy + g2 * tyz;
                 rv[3] = g0 * txz + g1 * tyz + g2 * tzz;
                 rv[4] = g0 * f4 + g1 * g4 + g2 * h4;
             }
-            if (face[u]) {
-                const double nrm = m.norm[(size_t)mo[u] * nfm + q * m.ms];
+            if (face) {
+                const double nrm = MF ? mf_nrm : m.norm[(size_t)mo * nfm + q * m.ms];
 #pragma unroll
-                for (int v = 0; v < 5; ++v) FV[(((size_t)e * 6 + fc[u] - 1) * NEQ + v) * S::NF + q] = rv[v] / nrm;
+                for (int v = 0; v < 5; ++v) FV[(((size_t)e * 6 + fc - 1) * NEQ + v) * S::NF + q] = rv[v] / nrm;
             } else {
                 double fi[5];
-                euler_from_primitive(pr[u], Qv[u], g0, g1, g2, fi);
+                euler_from_primitive(pr, Qv, g0, g1, g2, fi);
 #pragma unroll
                 for (int v = 0; v < 5; ++v) {
-                    if constexpr (PM == 0) sF[v * S::NS + s[u]] = fi[v] - rv[v] * ph.inv_re;
-                    else sF[v * S::NS + s[u]] = (CT)((fi[v] - rv[v] * ph.inv_re) - sBar[5 + D * 5 + v]);
+                    if constexpr (PM == 0) sF[v * S::NS + s] = fi[v] - rv[v] * ph.inv_re;
+                    else sF[v * S::NS + s] = (CT)((fi[v] - rv[v] * ph.inv_re) - sBar[5 + D * 5 + v]);
                 }
             }
         }
@@ -519,14 +611,14 @@ __device__ __forceinline__ void volume_grid(const DevMesh &m, const DevPhys &ph,
             const int b = D == 2 ? mi : l;
             CT Dk[S::NC];
 #pragma unroll
-            for (int j = 1; j < S::NC - 1; ++j) Dk[j] = opD<PM>(P, k * S::NC + j);
+            for (int j = 1; j < S::NC - 1; ++j) Dk[j] = sD[k * S::NC + j];
 #pragma unroll
             for (int v = 0; v < 5; ++v) {
                 CA t = 0;
 #pragma unroll
                 for (int j = 1; j < S::NC - 1; ++j) t = mac<PM>(Dk[j], sF[v * S::NS + sidx<P>(D, j, a, b)], t);
                 if constexpr (PM == 0) acc[r][v] -= t;
-                else acc[r][v] -= (double)t + sBar[5 + D * 5 + v] * c_Dint[P][k];
+                else acc[r][v] -= (double)t + sBar[5 + D * 5 + v] * sDint[k];
             }
         }
     }
@@ -561,48 +653,127 @@ __global__ __launch_bounds__(vblock<PM>()) void k_volume(const double *__restric
     constexpr int R = (S::NP + VBLOCK - 1) / VBLOCK;
     __shared__ CT sQ[NEQ * S::NP];            // physical Q at Gauss points (minus Qbar when mixed)
     __shared__ CT sG[15 * S::NP];             // dQ_v/dx_c at Gauss points, (c*5+v)
-    __shared__ CT sF[NEQ * S::NS];            // interior total flux of the current grid (minus Fbar)
+    // interior total flux of the current grid (minus Fbar); during the
+    // gradient phase it holds the face averages of the six faces
+    constexpr int NSF = NEQ * S::NS > 6 * NEQ * S::NF ? NEQ * S::NS : 6 * NEQ * S::NF;
+    __shared__ CT sF[NSF];
     __shared__ double sBar[5 + 15];           // mixed modes: Qbar[5], Fbar[3][5]
+    // operator matrices of this order: M = D_int B (Gauss derivative), D, B
+    constexpr int OM = 0, OD = S::NCG * S::NCG, OB = OD + S::NC * S::NC, NOP = OB + S::NC * S::NCG;
+    __shared__ CT sOp[NOP];
+    __shared__ double sDint[S::NCG];          // mixed modes: sum over interior j of D[k][j]
     const int e = blockIdx.x;
     const int tid = threadIdx.x;
     const double *jac = m.jacob + (size_t)e * (m.ms ? S::NP : 1);
-    if constexpr (PM != 0) {
-        constexpr int pc = (S::NCG / 2) * (1 + S::NCG + S::NCG * S::NCG);
-        if (tid < 5) sBar[tid] = JQ[((size_t)e * NEQ + tid) * S::NP + pc] * jac[pc * m.ms];
-        __syncthreads();
-        if (tid < 3) {
-            const int nsm = m.ms ? S::NS : 1;
-            const double *gmd = (tid == 0 ? m.gmet0 : (tid == 1 ? m.gmet1 : m.gmet2)) + (size_t)e * 3 * nsm;
-            double qb[5], fb[5];
-#pragma unroll
-            for (int v = 0; v < 5; ++v) qb[v] = sBar[v];
-            const Prim pb = primitives(ph, qb);
-            euler_from_primitive(pb, qb, gmd[0], gmd[nsm], gmd[2 * nsm], fb);
-#pragma unroll
-            for (int v = 0; v < 5; ++v) sBar[5 + tid * 5 + v] = fb[v];
-        }
-    }
-    {
-        const double *q = JQ + (size_t)e * NEQ * S::NP;
-        for (int i = tid; i < NEQ * S::NP; i += VBLOCK) {
-            const double qp = q[i] * jac[(i % S::NP) * m.ms];
-            if constexpr (PM == 0) sQ[i] = qp; else sQ[i] = (CT)(qp - sBar[i / S::NP]);
-        }
-    }
     const int *e2m = m.e2m + (size_t)e * 6;
     int mort[6];
 #pragma unroll
     for (int f = 0; f < 6; ++f) mort[f] = e2m[f] >> 1;
     const int npm = m.ms ? S::NP : 1;
+
+    // ---- prologue.  Every global load of the element's inputs is issued
+    // before the first wait: one exposed memory latency per element instead
+    // of one per loop trip (one workgroup per CU leaves nothing to hide it).
+    constexpr int NQV = NEQ * S::NP, NAV = 6 * NEQ * S::NF;
+    constexpr int NQI = (NQV + VBLOCK - 1) / VBLOCK, NAI = (NAV + VBLOCK - 1) / VBLOCK;
+    constexpr int NOI = (NOP + VBLOCK - 1) / VBLOCK;
+    double qv[NQI], jv[NQI], av[NAI];   // Q and 1/J kept apart: a product here would wait for both loads
+    CT ov[NOI];
+    CA gpre[R][9];   // metric rows of this work-item's Gauss points (gradient phase)
+    {
+        const double *q = JQ + (size_t)e * NEQ * S::NP;
+#pragma unroll
+        for (int u = 0; u < NQI; ++u) {
+            const int i = tid + u * VBLOCK;
+            qv[u] = i < NQV ? q[i] : 0.0;
+            jv[u] = i < NQV ? jac[(i % S::NP) * m.ms] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < NAI; ++u) {
+            const int i = tid + u * VBLOCK;
+            const int f = i / (NEQ * S::NF), vq = i % (NEQ * S::NF);
+            // mortar of face f+1 by selects (a dynamic index into mort[] would go to scratch)
+            int mo = mort[0];
+#pragma unroll
+            for (int g = 1; g < 6; ++g) mo = f == g ? mort[g] : mo;
+            av[u] = i < NAV ? QA[(size_t)mo * NEQ * S::NF + vq] : 0.0;
+        }
+        const double *gg = m.gmetg + (size_t)e * 9 * npm;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int pt = tid + r * VBLOCK;
+#pragma unroll
+            for (int i = 0; i < 9; ++i) gpre[r][i] = pt < S::NP ? (CA)gg[i * npm + pt * m.ms] : (CA)0;
+        }
+#pragma unroll
+        for (int u = 0; u < NOI; ++u) {
+            const int i = tid + u * VBLOCK;
+            // one unconditional load from a selected address (branches here made the compiler wait)
+            const int ic = i < NOP ? i : NOP - 1;
+            const CT *src;
+            if constexpr (PM == 0) src = ic < OD ? &c_M[P][ic - OM] : (ic < OB ? &c_D[P][ic - OD] : &c_B[P][ic - OB]);
+            else src = ic < OD ? &c_Mf[P][ic - OM] : (ic < OB ? &c_Df[P][ic - OD] : &c_Bf[P][ic - OB]);
+            ov[u] = *src;
+        }
+    }
+    constexpr bool MF = (PM == 0);
+    const MfLane<P> ml = mf_lane<P>(tid);
+    if constexpr (PM != 0) {
+        // Qbar = Q at the central Gauss point, Fbar_d = Euler flux of Qbar
+        // with the metric row at staggered point 0 of grid d
+        constexpr int pc = (S::NCG / 2) * (1 + S::NCG + S::NCG * S::NCG);
+        const int nsm = m.ms ? S::NS : 1;
+        double qc = 0.0, gm[3] = {0.0, 0.0, 0.0};
+        if (tid < 5) qc = JQ[((size_t)e * NEQ + tid) * S::NP + pc] * jac[pc * m.ms];
+        if (tid < 3) {
+            const double *gmd = (tid == 0 ? m.gmet0 : (tid == 1 ? m.gmet1 : m.gmet2)) + (size_t)e * 3 * nsm;
+#pragma unroll
+            for (int i = 0; i < 3; ++i) gm[i] = gmd[i * nsm];
+        }
+        if (tid < S::NCG) sDint[tid] = c_Dint[P][tid];
+        if (tid < 5) sBar[tid] = qc;
+        __syncthreads();
+        if (tid < 3) {
+            double qb[5], fb[5];
+#pragma unroll
+            for (int v = 0; v < 5; ++v) qb[v] = sBar[v];
+            const Prim pb = primitives(ph, qb);
+            euler_from_primitive(pb, qb, gm[0], gm[1], gm[2], fb);
+#pragma unroll
+            for (int v = 0; v < 5; ++v) sBar[5 + tid * 5 + v] = fb[v];
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < NQI; ++u) {
+        const int i = tid + u * VBLOCK;
+        if (i < NQV) {
+            const double qp = qv[u] * jv[u];
+            if constexpr (PM == 0) sQ[i] = qp; else sQ[i] = (CT)(qp - sBar[i / S::NP]);
+        }
+    }
+    // face averages of the six faces (face f+1 at [f][v][q]), minus Qbar when mixed
+    CT *sQA = sF;
+#pragma unroll
+    for (int u = 0; u < NAI; ++u) {
+        const int i = tid + u * VBLOCK;
+        if (i < NAV) {
+            if constexpr (PM == 0) sQA[i] = av[u]; else sQA[i] = (CT)(av[u] - sBar[(i % (NEQ * S::NF)) / S::NF]);
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < NOI; ++u) {
+        const int i = tid + u * VBLOCK;
+        if (i < NOP) sOp[i] = ov[u];
+    }
     __syncthreads();
 
     // ---- gradients: w_d = (D_int B) Q + D(:,0) QA_lo + D(:,nc-1) QA_hi, then metric combine
     {
-        const double *gg = m.gmetg + (size_t)e * 9 * npm;
-#pragma unroll 1
-        for (int pt = tid; pt < S::NP; pt += VBLOCK) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int pt = tid + r * VBLOCK;
+            if (pt >= S::NP) break;
             const int n = pt % S::NCG, mi = (pt / S::NCG) % S::NCG, l = pt / (S::NCG * S::NCG);
-            const int pq = pt * m.ms;
             CA out[3][5];
 #pragma unroll
             for (int d = 0; d < 3; ++d) {
@@ -612,25 +783,18 @@ __global__ __launch_bounds__(vblock<PM>()) void k_volume(const double *__restric
                 const int q = a + S::NCG * b;
                 CT Mk[S::NCG];
 #pragma unroll
-                for (int k2 = 0; k2 < S::NCG; ++k2) Mk[k2] = opM<PM>(P, k * S::NCG + k2);
-                const CT d0 = opD<PM>(P, k * S::NC), d1 = opD<PM>(P, k * S::NC + S::NC - 1);
-                const double *qa0 = QA + (size_t)mort[face_of<P>(d, 0) - 1] * NEQ * S::NF + q;
-                const double *qa1 = QA + (size_t)mort[face_of<P>(d, 1) - 1] * NEQ * S::NF + q;
-                CA gd[3];
-#pragma unroll
-                for (int c = 0; c < 3; ++c) gd[c] = (CA)gg[(d * 3 + c) * npm + pq];
+                for (int k2 = 0; k2 < S::NCG; ++k2) Mk[k2] = sOp[OM + k * S::NCG + k2];
+                const CT d0 = sOp[OD + k * S::NC], d1 = sOp[OD + k * S::NC + S::NC - 1];
+                const CT *qa0 = sQA + (face_of<P>(d, 0) - 1) * NEQ * S::NF + q;
+                const CT *qa1 = sQA + (face_of<P>(d, 1) - 1) * NEQ * S::NF + q;
+                const CA *gd = gpre[r] + d * 3;
 #pragma unroll
                 for (int v = 0; v < 5; ++v) {
                     CA t = 0;
 #pragma unroll
                     for (int k2 = 0; k2 < S::NCG; ++k2) t = mac<PM>(Mk[k2], sQ[v * S::NP + gidx<P>(d, k2, a, b)], t);
-                    if constexpr (PM == 0) {
-                        t = fma(d0, qa0[v * S::NF], t);
-                        t = fma(d1, qa1[v * S::NF], t);
-                    } else {
-                        t = mac<PM>(d0, (CT)(qa0[v * S::NF] - sBar[v]), t);
-                        t = mac<PM>(d1, (CT)(qa1[v * S::NF] - sBar[v]), t);
-                    }
+                    t = mac<PM>(d0, qa0[v * S::NF], t);
+                    t = mac<PM>(d1, qa1[v * S::NF], t);
 #pragma unroll
                     for (int c = 0; c < 3; ++c) out[c][v] = d == 0 ? gd[c] * t : fma(gd[c], t, out[c][v]);
                 }
@@ -648,12 +812,9 @@ __global__ __launch_bounds__(vblock<PM>()) void k_volume(const double *__restric
     for (int r = 0; r < R; ++r)
 #pragma unroll
         for (int v = 0; v < 5; ++v) acc[r][v] = 0.0;
-    // one staggered point per work-item (two per work-item, sharing the line
-    // loads, measured slower once the FP64 workgroup has 1024 work-items)
-    constexpr int NPT = 1;
-    volume_grid<P, 0, PM, NPT>(m, ph, QA, muh, nut, FV, Slag, sQ, sG, sF, sBar, e, mort, acc);
-    volume_grid<P, 1, PM, NPT>(m, ph, QA, muh, nut, FV, Slag, sQ, sG, sF, sBar, e, mort, acc);
-    volume_grid<P, 2, PM, NPT>(m, ph, QA, muh, nut, FV, Slag, sQ, sG, sF, sBar, e, mort, acc);
+    volume_grid<P, 0, PM, MF>(m, ph, QA, muh, nut, FV, Slag, sQ, sG, sF, sBar, sOp + OD, sOp + OB, sDint, e, mort, ml, acc);
+    volume_grid<P, 1, PM, MF>(m, ph, QA, muh, nut, FV, Slag, sQ, sG, sF, sBar, sOp + OD, sOp + OB, sDint, e, mort, ml, acc);
+    volume_grid<P, 2, PM, MF>(m, ph, QA, muh, nut, FV, Slag, sQ, sG, sF, sBar, sOp + OD, sOp + OB, sDint, e, mort, ml, acc);
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         const int pt = tid + r * VBLOCK;
