@@ -394,16 +394,20 @@ __device__ __forceinline__ MfLane<P> mf_lane(int tid)
 }
 
 // sum_k B[j][k] fld[k along the lane's line], for the point the lane owns;
-// off/ok: LDS offsets and validity of the Gauss values this lane supplies.
+// off: LDS offsets of the Gauss values this lane supplies.  Loads are
+// unconditional (a masked load becomes a branch with its own wait): a lane
+// whose Gauss index k lies past the line reads an in-range value that meets
+// a zero coefficient (A = 0 for k >= ncg), and a lane past the last line only
+// feeds outputs that no lane keeps.
 template <int P>
-__device__ __forceinline__ double mf_line(const double *fld, const MfLane<P> &ml, const int *off, const bool *ok)
+__device__ __forceinline__ double mf_line(const double *fld, const MfLane<P> &ml, const int *off)
 {
+    double bv[MfTiles<P>::KS];
+#pragma unroll
+    for (int ks = 0; ks < MfTiles<P>::KS; ++ks) bv[ks] = fld[off[ks]];
     double t = 0.0;
 #pragma unroll
-    for (int ks = 0; ks < MfTiles<P>::KS; ++ks) {
-        const double bv = ok[ks] ? fld[off[ks]] : 0.0;
-        t = __builtin_amdgcn_mfma_f64_4x4x4f64(ml.A[ks], bv, t, 0, 0, 0);
-    }
+    for (int ks = 0; ks < MfTiles<P>::KS; ++ks) t = __builtin_amdgcn_mfma_f64_4x4x4f64(ml.A[ks], bv[ks], t, 0, 0, 0);
     return t;
 }
 
@@ -470,7 +474,6 @@ __device__ __forceinline__ void volume_grid(const DevMesh &m, const DevPhys &ph,
         int a, b, j;
         bool valid = true;
         int off[KS];
-        bool ok[KS];
         if constexpr (MF) {
             j = ml.j;
             a = ml.line % S::NCG;
@@ -479,8 +482,7 @@ __device__ __forceinline__ void volume_grid(const DevMesh &m, const DevPhys &ph,
 #pragma unroll
             for (int ks = 0; ks < KS; ++ks) {
                 const int k = 4 * ks + ml.r;
-                ok[ks] = ml.lvalid && k < S::NCG;
-                off[ks] = ok[ks] ? gidx<P>(D, k, a, b) : 0;
+                off[ks] = gidx<P>(D, k < S::NCG ? k : S::NCG - 1, a, b);   // in range (ml.line is 0 past the last line)
             }
         } else {
             const int s0 = w;
@@ -499,7 +501,7 @@ __device__ __forceinline__ void volume_grid(const DevMesh &m, const DevPhys &ph,
         if constexpr (MF) {
 #pragma unroll
             for (int v = 0; v < 5; ++v) {
-                const double t = mf_line<P>(sQ + v * S::NP, ml, off, ok);
+                const double t = mf_line<P>(sQ + v * S::NP, ml, off);
                 Qv[v] = face ? mf_qa[v] : t;
             }
         } else {
@@ -539,7 +541,7 @@ __device__ __forceinline__ void volume_grid(const DevMesh &m, const DevPhys &ph,
             for (int v = 0; v < 5; ++v) {
                 const CT *gl = sG + (c * 5 + v) * S::NP;
                 if constexpr (MF) {
-                    Gc[v] = mf_line<P>(gl, ml, off, ok);
+                    Gc[v] = mf_line<P>(gl, ml, off);
                 } else {
                     CA u = 0;
 #pragma unroll
