@@ -867,6 +867,33 @@ struct StageArgs {
 constexpr int UBLOCK = 512;
 constexpr int UBLK = 4;
 
+// One batch of UBLK points of k_update: Qtp, and for an RK stage G, JQ and
+// (with traces) 1/J.  Indices past the element are clamped in range; the
+// caller skips those points.
+struct UpdBatch {
+    double q[UBLK], g[UBLK], j[UBLK], c[UBLK];
+};
+
+template <int P, bool TRACES>
+__device__ __forceinline__ UpdBatch upd_load(const double *__restrict__ Qtp, const double *__restrict__ G,
+                                             const double *__restrict__ JQ, const double *__restrict__ jac, int ms,
+                                             const StageArgs &st, size_t e0, int base)
+{
+    using S = Sz<P>;
+    constexpr int NQ = NEQ * S::NP;
+    UpdBatch bt;
+#pragma unroll
+    for (int u = 0; u < UBLK; ++u) {
+        const int i = base + u * UBLOCK;
+        const int ic = i < NQ ? i : NQ - 1;
+        bt.q[u] = Qtp[e0 + ic];
+        bt.g[u] = (st.rk && !st.first) ? G[e0 + ic] : 0.0;
+        bt.j[u] = st.rk ? JQ[e0 + ic] : 0.0;
+        bt.c[u] = (st.rk && TRACES) ? jac[(ic % S::NP) * ms] : 0.0;
+    }
+    return bt;
+}
+
 template <int P, bool TRACES>
 __global__ __launch_bounds__(UBLOCK) void k_update(const double *__restrict__ Qtp, const double *__restrict__ FT, DevMesh m,
                                                     StageArgs st, double *__restrict__ JQ, double *__restrict__ G,
@@ -874,54 +901,68 @@ __global__ __launch_bounds__(UBLOCK) void k_update(const double *__restrict__ Qt
 {
     using S = Sz<P>;
     constexpr int NQ = NEQ * S::NP, NFT = 6 * NEQ * S::NF;
+    constexpr int NFI = (NFT + UBLOCK - 1) / UBLOCK;            // face values per work-item
+    constexpr int NTRIP = (NQ + UBLOCK * UBLK - 1) / (UBLOCK * UBLK);
     __shared__ double sFT[NFT];
     __shared__ double sQ[TRACES ? NQ : 1];
+    __shared__ double sDf[2 * S::NCG];   // D[k][0], D[k][nc-1]: face part of the divergence
+    __shared__ double sBf[2 * S::NCG];   // B[0][k], B[nc-1][k]: face traces
     const int e = blockIdx.x, tid = threadIdx.x;
+    // Mortar and orientation sign of the six faces: uniform per element, so
+    // these are scalar loads; each work-item picks its face by selects.
     const int *e2m = m.e2m + (size_t)e * 6;
-    for (int base = tid; base < NFT; base += UBLOCK * UBLK) {
-        double val[UBLK];
+    int fmo[6];
+    double fsg[6];
 #pragma unroll
-        for (int u = 0; u < UBLK; ++u) {
-            const int i = base + u * UBLOCK;
-            if (i < NFT) {
-                const int f = i / (NEQ * S::NF), r = i % (NEQ * S::NF);
-                const int c = e2m[f];
-                val[u] = FT[(size_t)(c >> 1) * NEQ * S::NF + r];
-                if ((c & 1) && m.m_sign[c >> 1] == -1) val[u] = -val[u];
-            }
-        }
-#pragma unroll
-        for (int u = 0; u < UBLK; ++u) {
-            const int i = base + u * UBLOCK;
-            if (i < NFT) sFT[i] = val[u];
-        }
+    for (int f = 0; f < 6; ++f) {
+        const int c = e2m[f];
+        fmo[f] = c >> 1;
+        fsg[f] = ((c & 1) && m.m_sign[c >> 1] == -1) ? -1.0 : 1.0;
     }
-    __syncthreads();
+    // Every global load of the face fluxes and of the first batch of points
+    // is issued before the first wait.
+    double fv[NFI];
+#pragma unroll
+    for (int u = 0; u < NFI; ++u) {
+        const int i = tid + u * UBLOCK;
+        const int f = i / (NEQ * S::NF), r = i % (NEQ * S::NF);
+        int mo = fmo[0];
+#pragma unroll
+        for (int g = 1; g < 6; ++g) mo = f == g ? fmo[g] : mo;
+        fv[u] = i < NFT ? FT[(size_t)mo * NEQ * S::NF + r] : 0.0;
+    }
     const double *jac = m.jacob + (size_t)e * (m.ms ? S::NP : 1);
     const size_t e0 = (size_t)e * NQ;
+    UpdBatch bt = upd_load<P, TRACES>(Qtp, G, JQ, jac, m.ms, st, e0, tid);
+    if (tid < S::NCG) {
+        sDf[tid] = c_D[P][tid * S::NC];
+        sDf[S::NCG + tid] = c_D[P][tid * S::NC + S::NC - 1];
+        sBf[tid] = c_B[P][tid];
+        sBf[S::NCG + tid] = c_B[P][(S::NC - 1) * S::NCG + tid];
+    }
+#pragma unroll
+    for (int u = 0; u < NFI; ++u) {
+        const int i = tid + u * UBLOCK;
+        const int f = i / (NEQ * S::NF);
+        double sg = fsg[0];
+#pragma unroll
+        for (int g = 1; g < 6; ++g) sg = f == g ? fsg[g] : sg;
+        if (i < NFT) sFT[i] = sg < 0.0 ? -fv[u] : fv[u];
+    }
+    __syncthreads();
     double mx = 0.0;
     bool bad = false;
-    for (int base = tid; base < NQ; base += UBLOCK * UBLK) {
-        double qv[UBLK], gv[UBLK], jv[UBLK], jc[UBLK];
 #pragma unroll
-        for (int u = 0; u < UBLK; ++u) {
-            const int i = base + u * UBLOCK;
-            if (i < NQ) {
-                qv[u] = Qtp[e0 + i];
-                if (st.rk) {
-                    gv[u] = st.first ? 0.0 : G[e0 + i];
-                    jv[u] = JQ[e0 + i];
-                    if (TRACES) jc[u] = jac[(i % S::NP) * m.ms];
-                }
-            }
-        }
+    for (int trip = 0; trip < NTRIP; ++trip) {
+        const int base = tid + trip * UBLOCK * UBLK;
+        if (trip > 0) bt = upd_load<P, TRACES>(Qtp, G, JQ, jac, m.ms, st, e0, base);
 #pragma unroll
         for (int u = 0; u < UBLK; ++u) {
             const int i = base + u * UBLOCK;
             if (i >= NQ) continue;
             const int v = i / S::NP, pt = i % S::NP;
             const int n = pt % S::NCG, mi = (pt / S::NCG) % S::NCG, l = pt / (S::NCG * S::NCG);
-            double qt = qv[u];
+            double qt = bt.q[u];
 #pragma unroll
             for (int d = 0; d < 3; ++d) {
                 const int k = d == 0 ? n : (d == 1 ? mi : l);
@@ -930,14 +971,14 @@ __global__ __launch_bounds__(UBLOCK) void k_update(const double *__restrict__ Qt
                 const int q = a + S::NCG * b;
                 const double F0 = sFT[((face_of<P>(d, 0) - 1) * NEQ + v) * S::NF + q];
                 const double F1 = sFT[((face_of<P>(d, 1) - 1) * NEQ + v) * S::NF + q];
-                qt = qt - fma(c_D[P][k * S::NC + 0], F0, c_D[P][k * S::NC + S::NC - 1] * F1);
+                qt = qt - fma(sDf[k], F0, sDf[S::NCG + k] * F1);
             }
             if (st.rk) {
-                const double g = st.first ? qt : fma(st.a, gv[u], qt);
+                const double g = st.first ? qt : fma(st.a, bt.g[u], qt);
                 G[e0 + i] = g;
-                const double jq = fma(st.cdt, g, jv[u]);
+                const double jq = fma(st.cdt, g, bt.j[u]);
                 JQ[e0 + i] = jq;
-                if (TRACES) sQ[i] = jq * jc[u];
+                if (TRACES) sQ[i] = jq * bt.c[u];
             } else {
                 Qt[e0 + i] = qt;
             }
@@ -954,10 +995,10 @@ __global__ __launch_bounds__(UBLOCK) void k_update(const double *__restrict__ Qt
             const int qf = i % S::NF, v = (i / S::NF) % NEQ, f = i / (NEQ * S::NF) + 1;
             const int a = qf % S::NCG, b = qf / S::NCG;
             const int d = (f == 4 || f == 6) ? 0 : ((f == 1 || f == 2) ? 1 : 2);
-            const int j = (f == 2 || f == 4 || f == 5) ? S::NC - 1 : 0;
+            const double *Bj = sBf + ((f == 2 || f == 4 || f == 5) ? S::NCG : 0);
             double s2 = 0.0;
 #pragma unroll
-            for (int k = 0; k < S::NCG; ++k) s2 = fma(c_B[P][j * S::NCG + k], sQ[v * S::NP + gidx<P>(d, k, a, b)], s2);
+            for (int k = 0; k < S::NCG; ++k) s2 = fma(Bj[k], sQ[v * S::NP + gidx<P>(d, k, a, b)], s2);
             t[i] = s2;
         }
     }
