@@ -531,10 +531,13 @@ __device__ __forceinline__ void volume_grid(const DevMesh &m, const DevPhys &ph,
         // Velocity and temperature gradients, one direction c at a time: the
         // prolonged conservative gradient dQ/dx_c (5 fields) is turned into
         // (du/dx_c, dv/dx_c, dw/dx_c, dT/dx_c) and folded into the symmetric
-        // velocity-gradient sums, so no more than one direction's loads are live.
+        // velocity-gradient sums.  Unrolled in the matrix-core form (the next
+        // direction's loads overlap this one's arithmetic, measured 8 % faster);
+        // one direction at a time in the vector form, whose registers would spill.
         double Sxx = 0.0, Syy = 0.0, Szz = 0.0, Sxy = 0.0, Sxz = 0.0, Syz = 0.0, div = 0.0;
         double Tx = 0.0, Ty = 0.0, Tz = 0.0;
-#pragma unroll 1
+        constexpr int CUNROLL = MF ? 3 : 1;
+#pragma unroll CUNROLL
         for (int c = 0; c < 3; ++c) {
             double Gc[5];
 #pragma unroll
