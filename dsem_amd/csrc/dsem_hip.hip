@@ -341,40 +341,41 @@ __device__ __forceinline__ void load_physical_state(const double *__restrict__ J
     for (int i = threadIdx.x; i < NEQ * S::NP; i += BLOCK) sQ[i] = q[i] * jac[(i % S::NP) * m.ms];
 }
 
-// Line contractions of the FP64 staggered pass on the matrix cores.
+// Line contractions of the FP64 volume kernel on the matrix cores.
 // v_mfma_f64_4x4x4f64 runs four independent 4x4x4 products per wave.  Lane l
 // (c = l%4, blk = (l/4)%4, r = l/16) supplies A_blk[c][r] and B_blk[r][c]
 // and receives D_blk[r][c] (layout measured on gfx950).  With A = four rows
-// of the Gauss->Lobatto interpolation matrix B and B = four Gauss values of
-// four lines, D[r][c] is the value at Lobatto index 4 jt + r of line 4 g + c,
-// the point this lane owns: the pointwise pipeline that follows runs on it
-// unchanged, and each Gauss value is read from LDS once per 4x4 tile instead
-// of once per output point.  A grid of nc x ncg^2 points is 4x4 tiles over
-// (Lobatto index, line); tile t = 4 * wave + blk, t = g * JT + jt.
-template <int P>
+// of a ROWS x ncg line operator (the Gauss->Lobatto interpolation B, ROWS =
+// nc, or the Gauss derivative M = D_int B, ROWS = ncg) and B = four Gauss
+// values of four lines, D[r][c] is the result at row 4 jt + r of line 4 g + c,
+// the point this lane owns: the pointwise work that follows runs on it
+// without staging, and each Gauss value is read from LDS once per 4x4 tile
+// instead of once per output point.  ROWS x ncg^2 points form 4x4 tiles over
+// (row, line); tile t = 4 * wave + blk, t = g * JT + jt.
+template <int P, int ROWS = Sz<P>::NC>
 struct MfTiles {
     using S = Sz<P>;
-    static constexpr int JT = (S::NC + 3) / 4;     // tiles along the Lobatto index
+    static constexpr int JT = (ROWS + 3) / 4;      // tiles along the row index
     static constexpr int KS = (S::NCG + 3) / 4;    // steps of 4 Gauss points
-    static constexpr int NL = S::NCG * S::NCG;     // lines per grid
-    static constexpr int NT = ((NL + 3) / 4) * JT; // tiles per grid
+    static constexpr int NL = S::NCG * S::NCG;     // lines per direction
+    static constexpr int NT = ((NL + 3) / 4) * JT; // tiles per direction
 };
 
-template <int P>
+template <int P, int ROWS = Sz<P>::NC>
 struct MfLane {
-    double A[MfTiles<P>::KS];   // B[4 jt + c][4 ks + r], zero outside B
-    int r, j, line;             // line is also the line this lane loads
-    bool lvalid;                // tile and line exist (the lane loads)
-    bool valid;                 // and its Lobatto index exists (the lane owns a point)
-    bool wave_active;           // some tile of this wave exists (uniform per wave)
+    double A[MfTiles<P, ROWS>::KS];   // op[4 jt + c][4 ks + r], zero outside the operator
+    int r, j, line;                   // line is also the line this lane loads
+    bool lvalid;                      // tile and line exist (the lane loads)
+    bool valid;                       // and its row exists (the lane owns a point)
+    bool wave_active;                 // some tile of this wave exists (uniform per wave)
 };
 
-template <int P>
-__device__ __forceinline__ MfLane<P> mf_lane(int tid)
+template <int P, int ROWS = Sz<P>::NC>
+__device__ __forceinline__ MfLane<P, ROWS> mf_lane(int tid, const double *op)
 {
     using S = Sz<P>;
-    using T = MfTiles<P>;
-    MfLane<P> ml;
+    using T = MfTiles<P, ROWS>;
+    MfLane<P, ROWS> ml;
     const int l = tid % 64, c = l % 4, blk = (l / 4) % 4;
     ml.r = l / 16;
     const int t = (tid / 64) * 4 + blk;
@@ -382,32 +383,33 @@ __device__ __forceinline__ MfLane<P> mf_lane(int tid)
     ml.j = 4 * jt + ml.r;
     ml.line = 4 * g + c;
     ml.lvalid = t < T::NT && ml.line < T::NL;
-    ml.valid = ml.lvalid && ml.j < S::NC;
+    ml.valid = ml.lvalid && ml.j < ROWS;
     ml.wave_active = (tid / 64) * 4 < T::NT;
     if (!ml.lvalid) ml.line = 0;
 #pragma unroll
     for (int ks = 0; ks < T::KS; ++ks) {
         const int row = 4 * jt + c, col = 4 * ks + ml.r;
-        ml.A[ks] = (row < S::NC && col < S::NCG) ? c_B[P][row * S::NCG + col] : 0.0;
+        ml.A[ks] = (row < ROWS && col < S::NCG) ? op[row * S::NCG + col] : 0.0;
     }
     return ml;
 }
 
-// sum_k B[j][k] fld[k along the lane's line], for the point the lane owns;
+// sum_k op[j][k] fld[k along the lane's line], for the point the lane owns;
 // off: LDS offsets of the Gauss values this lane supplies.  Loads are
 // unconditional (a masked load becomes a branch with its own wait): a lane
 // whose Gauss index k lies past the line reads an in-range value that meets
 // a zero coefficient (A = 0 for k >= ncg), and a lane past the last line only
 // feeds outputs that no lane keeps.
-template <int P>
-__device__ __forceinline__ double mf_line(const double *fld, const MfLane<P> &ml, const int *off)
+template <int P, int ROWS>
+__device__ __forceinline__ double mf_line(const double *fld, const MfLane<P, ROWS> &ml, const int *off)
 {
-    double bv[MfTiles<P>::KS];
+    constexpr int KS = MfTiles<P, ROWS>::KS;
+    double bv[KS];
 #pragma unroll
-    for (int ks = 0; ks < MfTiles<P>::KS; ++ks) bv[ks] = fld[off[ks]];
+    for (int ks = 0; ks < KS; ++ks) bv[ks] = fld[off[ks]];
     double t = 0.0;
 #pragma unroll
-    for (int ks = 0; ks < MfTiles<P>::KS; ++ks) t = __builtin_amdgcn_mfma_f64_4x4x4f64(ml.A[ks], bv[ks], t, 0, 0, 0);
+    for (int ks = 0; ks < KS; ++ks) t = __builtin_amdgcn_mfma_f64_4x4x4f64(ml.A[ks], bv[ks], t, 0, 0, 0);
     return t;
 }
 
@@ -591,9 +593,11 @@ __device__ __forceinline__ void volume_grid(const DevMesh &m, const DevPhys &ph,
                 rv[4] = g0 * f4 + g1 * g4 + g2 * h4;
             }
             if (face) {
-                const double nrm = MF ? mf_nrm : m.norm[(size_t)mo * nfm + q * m.ms];
+                // r / norm as one reciprocal and five products (a rounding apart
+                // from five divisions, which every wave with a face lane paid)
+                const double rn = 1.0 / (MF ? mf_nrm : m.norm[(size_t)mo * nfm + q * m.ms]);
 #pragma unroll
-                for (int v = 0; v < 5; ++v) FV[(((size_t)e * 6 + fc - 1) * NEQ + v) * S::NF + q] = rv[v] / nrm;
+                for (int v = 0; v < 5; ++v) FV[(((size_t)e * 6 + fc - 1) * NEQ + v) * S::NF + q] = rv[v] * rn;
             } else {
                 double fi[5];
                 euler_from_primitive(pr, Qv, g0, g1, g2, fi);
@@ -722,7 +726,8 @@ __global__ __launch_bounds__(vblock<PM>()) void k_volume(const double *__restric
         }
     }
     constexpr bool MF = (PM == 0);
-    const MfLane<P> ml = mf_lane<P>(tid);
+    const MfLane<P> ml = mf_lane<P>(tid, c_B[P]);
+    const MfLane<P, S::NCG> mg = mf_lane<P, S::NCG>(tid, c_M[P]);   // gradient tiles (FP64 form)
     if constexpr (PM != 0) {
         // Qbar = Q at the central Gauss point, Fbar_d = Euler flux of Qbar
         // with the metric row at staggered point 0 of grid d
@@ -773,7 +778,56 @@ __global__ __launch_bounds__(vblock<PM>()) void k_volume(const double *__restric
     __syncthreads();
 
     // ---- gradients: w_d = (D_int B) Q + D(:,0) QA_lo + D(:,nc-1) QA_hi, then metric combine
-    {
+    if constexpr (MF) {
+        // Matrix-core form.  Direction by direction, the tile lane of each
+        // Gauss point writes w_d (5 fields) into slots d*5+v of sG; one
+        // direction's tiles cover every point once, so the three directions
+        // need no barrier between them.  Then each work-item combines the 15
+        // w values of its point with the metric rows, in place, in the same
+        // order as the vector form: g0 w0, then + g1 w1, then + g2 w2.
+        constexpr int KSG = MfTiles<P, S::NCG>::KS;
+        static_assert(MfTiles<P, S::NCG>::NT <= VBLOCK / 16, "one gradient tile per 16 lanes in one pass");
+        if (mg.wave_active) {
+            const int a = mg.line % S::NCG, b = mg.line / S::NCG;   // mg.line = a + ncg b
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                int off[KSG];
+#pragma unroll
+                for (int ks = 0; ks < KSG; ++ks) {
+                    const int k = 4 * ks + mg.r;
+                    off[ks] = gidx<P>(d, k < S::NCG ? k : S::NCG - 1, a, b);
+                }
+                double w[5];
+#pragma unroll
+                for (int v = 0; v < 5; ++v) w[v] = mf_line<P, S::NCG>(sQ + v * S::NP, mg, off);
+                if (mg.valid) {
+                    const int k = mg.j;
+                    const double d0 = sOp[OD + k * S::NC], d1 = sOp[OD + k * S::NC + S::NC - 1];
+                    const double *qa0 = sQA + (face_of<P>(d, 0) - 1) * NEQ * S::NF + mg.line;
+                    const double *qa1 = sQA + (face_of<P>(d, 1) - 1) * NEQ * S::NF + mg.line;
+                    const int pt = gidx<P>(d, k, a, b);
+#pragma unroll
+                    for (int v = 0; v < 5; ++v)
+                        sG[(d * 5 + v) * S::NP + pt] = fma(d1, qa1[v * S::NF], fma(d0, qa0[v * S::NF], w[v]));
+                }
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int pt = tid + r * VBLOCK;
+            if (pt >= S::NP) break;
+            double w[15];
+#pragma unroll
+            for (int i = 0; i < 15; ++i) w[i] = sG[i * S::NP + pt];
+            const CA *g = gpre[r];
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+#pragma unroll
+                for (int v = 0; v < 5; ++v)
+                    sG[(c * 5 + v) * S::NP + pt] = fma(g[6 + c], w[10 + v], fma(g[3 + c], w[5 + v], g[c] * w[v]));
+        }
+    } else {
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             const int pt = tid + r * VBLOCK;
