@@ -464,7 +464,15 @@ __device__ __forceinline__ void volume_grid(const DevMesh &m, const DevPhys &ph,
             }
         }
     }
-    __syncthreads();   // sG complete / previous divergence pass done with sF
+    // Matrix-core form, grids 1 and 2: a lane's flux waits in registers, and the
+    // barrier that keeps the previous divergence pass's reads of sF ahead of
+    // this grid's writes comes after the arithmetic, so the two passes overlap
+    // across waves.  Grid 0 (and the vector form) waits here: sG complete /
+    // previous divergence pass done with sF.
+    double mf_f[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    int mf_fs = 0;
+    bool mf_fstore = false;
+    if constexpr (!MF || D == 0) __syncthreads();
     // Matrix-core form: one pass, every lane of an active wave takes part in
     // the contractions (the tiles need all 64 lanes); only lanes that own a
     // point store.  Vector form: work-items stride over the points.
@@ -601,15 +609,29 @@ __device__ __forceinline__ void volume_grid(const DevMesh &m, const DevPhys &ph,
             } else {
                 double fi[5];
                 euler_from_primitive(pr, Qv, g0, g1, g2, fi);
+                if constexpr (MF) {
 #pragma unroll
-                for (int v = 0; v < 5; ++v) {
-                    if constexpr (PM == 0) sF[v * S::NS + s] = fi[v] - rv[v] * ph.inv_re;
-                    else sF[v * S::NS + s] = (CT)((fi[v] - rv[v] * ph.inv_re) - sBar[5 + D * 5 + v]);
+                    for (int v = 0; v < 5; ++v) mf_f[v] = fi[v] - rv[v] * ph.inv_re;
+                    mf_fs = s;
+                    mf_fstore = true;
+                } else {
+#pragma unroll
+                    for (int v = 0; v < 5; ++v) {
+                        if constexpr (PM == 0) sF[v * S::NS + s] = fi[v] - rv[v] * ph.inv_re;
+                        else sF[v * S::NS + s] = (CT)((fi[v] - rv[v] * ph.inv_re) - sBar[5 + D * 5 + v]);
+                    }
                 }
             }
         }
     }
-    __syncthreads();
+    if constexpr (MF) {
+        if constexpr (D > 0) __syncthreads();   // every wave is past the previous divergence pass
+        if (mf_fstore) {
+#pragma unroll
+            for (int v = 0; v < 5; ++v) sF[v * S::NS + mf_fs] = mf_f[v];
+        }
+    }
+    __syncthreads();   // fluxes of this grid complete
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         const int pt = tid + r * VBLOCK;
