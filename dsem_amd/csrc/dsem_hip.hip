@@ -690,7 +690,7 @@ __global__ __launch_bounds__(vblock<PM>()) void k_volume(const double *__restric
     __shared__ CT sF[NSF];
     __shared__ double sBar[5 + 15];           // mixed modes: Qbar[5], Fbar[3][5]
     // operator matrices of this order: M = D_int B (Gauss derivative), D, B
-    constexpr int OM = 0, OD = S::NCG * S::NCG, OB = OD + S::NC * S::NC, NOP = OB + S::NC * S::NCG;
+    constexpr int OM = 0, OD = S::NCG * S::NCG, OB = OD + S::NCG * S::NC, NOP = OB + S::NC * S::NCG;
     __shared__ CT sOp[NOP];
     __shared__ double sDint[S::NCG];          // mixed modes: sum over interior j of D[k][j]
     const int e = blockIdx.x;
