@@ -1540,6 +1540,7 @@ struct dsem_ctx {
     double *Slag = nullptr, *muhg = nullptr, *ev_resid = nullptr, *ev_ducros = nullptr;   // entropy viscosity
     bool have_lag = false;
     bool keep_lag = false;                     // save the last stage's staggered states without EV
+    bool traces_cur = false;                   // T holds the face traces of the current JQ
     bool les = false;                          // static Smagorinsky model
     double les_cs = 0.0, les_numax = 0.0;
     double *nut = nullptr, *nut_g = nullptr;   // eddy viscosity on the staggered grids / Gauss points
@@ -1665,6 +1666,7 @@ int run_rhs(dsem_ctx *c, double *JQ, const StageArgs &st, double *Qt_out, bool t
     const double *muh = c->params.shock ? c->muh : nullptr;
     const double *nut = c->les ? c->nut : nullptr;
     hipStream_t s = c->stream;
+    c->traces_cur = false;   // only dsem_rk_steps vouches for T after its last stage
     if (!traces_ready) TRY(launch(c, KID_TRACES, [&] { k_traces<P><<<K, BLOCK, 0, s>>>(JQ, m, c->T); }));
     TRY(launch(c, KID_FACE_INV, [&] { k_face_inv<P><<<face_blocks, BLOCK, 0, s>>>(c->T, m, ph, c->FI, c->QA); }));
     double *slag = save_lag ? c->Slag : nullptr;
@@ -2168,6 +2170,7 @@ int dsem_set_state(dsem_ctx *c, const double *JQ, int layout)
         src = tmp.data();
     }
     HIP_TRY(c, hipSetDevice(c->device));
+    c->traces_cur = false;
     HIP_TRY(c, hipMemcpyAsync(c->JQ, src, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     c->have_lag = false;   // a new state starts without lagged staggered states
@@ -2367,12 +2370,16 @@ int dsem_rk_steps(dsem_ctx *c, double t, double dt, int nsteps, double *resid_ou
             const bool last = check && s == nsteps - 1 && k == 4;
             st.resid = last ? &c->d_scalars[0] : nullptr;
             st.nonfinite = check ? reinterpret_cast<unsigned int *>(&c->d_scalars[2]) : nullptr;
-            const bool ready = !(s == 0 && k == 0);
-            const bool make = !(s == nsteps - 1 && k == 4);
-            TRY(rhs_dispatch(c, c->JQ, st, nullptr, ready, make, (c->params.shock || c->keep_lag || c->les) && k == 4));
+            // The first stage needs the traces of the state: left in T by the
+            // previous call's last stage unless the state or T changed since.
+            // Every stage's update, the last included, leaves the traces of the
+            // updated state for the next stage or call.
+            const bool ready = (s == 0 && k == 0) ? c->traces_cur : true;
+            TRY(rhs_dispatch(c, c->JQ, st, nullptr, ready, true, (c->params.shock || c->keep_lag || c->les) && k == 4));
         }
         if (c->params.shock || c->keep_lag || c->les) c->have_lag = true;
     }
+    c->traces_cur = true;
     if (check) {
         unsigned long long h[4];
         HIP_TRY(c, hipMemcpyAsync(h, c->d_scalars, sizeof(h), hipMemcpyDeviceToHost, c->stream));
@@ -2438,6 +2445,7 @@ int dsem_rk_step_vjp_device(dsem_ctx *c, double dt, double *dQbar, double *nubar
     double *S0;
     FwdTape f;
     tape_stage(c, 0, 0, &S0, &f);
+    c->traces_cur = false;
     HIP_TRY(c, hipMemcpyAsync(c->JQ, S0, sizeof(double) * (size_t)c->K * NEQ * c->np, hipMemcpyDeviceToDevice,
                               c->stream));   // the start state stays the context's state
     return vjp_recorded_step(c, dt, 0, dQbar, nubar);
@@ -2482,6 +2490,7 @@ int dsem_set_state_device(dsem_ctx *c, const double *dJQ)
 {
     if (!c || !c->stream || !dJQ) return set_err(c, DSEM_ERR_ARG, "set_state_device arguments");
     HIP_TRY(c, hipSetDevice(c->device));
+    c->traces_cur = false;
     HIP_TRY(c, hipMemcpyAsync(c->JQ, dJQ, sizeof(double) * (size_t)c->K * NEQ * c->np, hipMemcpyDeviceToDevice,
                               c->stream));
     HIP_TRY(c, hipStreamSynchronize(c->stream));

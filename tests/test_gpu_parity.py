@@ -195,3 +195,33 @@ def test_rhs_distorted_mesh_every_order(p):
     err = rel_l2(got, want)
     print("rel L2 per field", err, flush=True)
     assert np.all(err <= TOL), err
+
+
+def test_step_calls_reuse_face_traces():
+    """A call's last stage leaves the face traces of the new state for the
+    next call.  n one-step calls must then equal one n-step call bit for bit,
+    with rhs() and a new state in between invalidating the kept traces."""
+    print("\n[case] step calls vs one multi-step call 3^3 p=5", flush=True)
+    mesh = distorted_box(3, 5, amp=0.05, seed=4)
+    g, JQ = _case(mesh)
+    dt = 0.3 * O.dtcalc(g, JQ, O.phys())
+    with DSEM(mesh, metrics="per_point") as a, DSEM(mesh, metrics="per_point") as b:
+        a.set_state(JQ)
+        a.steps(3, dt)
+        b.set_state(JQ)
+        for _ in range(3):
+            b.tstep(0.0, dt)
+        assert np.array_equal(a.get_state(), b.get_state())
+        # rhs() in between: the step after it still equals a fresh context's
+        b.rhs()
+        b.tstep(0.0, dt)
+        a.steps(1, dt)
+        assert np.array_equal(a.get_state(), b.get_state())
+        # a new state in between: the kept traces belong to the old one
+        JQ2 = JQ * (1.0 + 1e-3 * np.random.default_rng(3).standard_normal(JQ.shape))
+        b.set_state(JQ2)
+        b.tstep(0.0, dt)
+        with DSEM(mesh, metrics="per_point") as c:
+            c.set_state(JQ2)
+            c.tstep(0.0, dt)
+            assert np.array_equal(b.get_state(), c.get_state())
