@@ -279,25 +279,47 @@ __device__ __forceinline__ void roe_flux(const DevPhys &ph, double n0, double n1
 }
 
 // ------------------------------------------------------------ kernels
+// Face traces of the physical state (Int_fluxes face values, send_q_to_mortars):
+// the element's state is staged in LDS with batched loads, then each trace
+// is a 9-term sum over LDS.
 template <int P>
 __global__ __launch_bounds__(BLOCK) void k_traces(const double *__restrict__ JQ, DevMesh m, double *__restrict__ T)
 {
     using S = Sz<P>;
+    constexpr int NQ = NEQ * S::NP, TB = 8;   // loads in flight per work-item
+    __shared__ double sQ[NQ];
+    __shared__ double sBf[2 * S::NCG];   // B[0][k], B[nc-1][k]
     const int e = blockIdx.x;
-    const double *q = JQ + (size_t)e * NEQ * S::NP;
+    const double *q = JQ + (size_t)e * NQ;
     const double *jac = m.jacob + (size_t)e * (m.ms ? S::NP : 1);
+    if (threadIdx.x < S::NCG) {
+        sBf[threadIdx.x] = c_B[P][threadIdx.x];
+        sBf[S::NCG + threadIdx.x] = c_B[P][(S::NC - 1) * S::NCG + threadIdx.x];
+    }
+    for (int i0 = threadIdx.x; i0 < NQ; i0 += BLOCK * TB) {
+        double qv[TB], jv[TB];
+#pragma unroll
+        for (int u = 0; u < TB; ++u) {
+            const int i = i0 + u * BLOCK, ic = i < NQ ? i : NQ - 1;
+            qv[u] = q[ic];
+            jv[u] = jac[(ic % S::NP) * m.ms];
+        }
+#pragma unroll
+        for (int u = 0; u < TB; ++u) {
+            const int i = i0 + u * BLOCK;
+            if (i < NQ) sQ[i] = qv[u] * jv[u];
+        }
+    }
+    __syncthreads();
     double *t = T + (size_t)e * 6 * NEQ * S::NF;
     for (int i = threadIdx.x; i < 6 * NEQ * S::NF; i += BLOCK) {
         const int qf = i % S::NF, v = (i / S::NF) % NEQ, f = i / (NEQ * S::NF) + 1;
         const int a = qf % S::NCG, b = qf / S::NCG;
         const int d = (f == 4 || f == 6) ? 0 : ((f == 1 || f == 2) ? 1 : 2);
-        const int j = (f == 2 || f == 4 || f == 5) ? S::NC - 1 : 0;
+        const double *Bj = sBf + ((f == 2 || f == 4 || f == 5) ? S::NCG : 0);
         double s = 0.0;
 #pragma unroll
-        for (int k = 0; k < S::NCG; ++k) {
-            const int pt = gidx<P>(d, k, a, b);
-            s = fma(c_B[P][j * S::NCG + k], q[v * S::NP + pt] * jac[pt * m.ms], s);
-        }
+        for (int k = 0; k < S::NCG; ++k) s = fma(Bj[k], sQ[v * S::NP + gidx<P>(d, k, a, b)], s);
         t[i] = s;
     }
 }
