@@ -94,6 +94,15 @@ class DSEM:
         self.close()
 
     # ------------------------------------------------------------------ data
+    def write_tecplot(self, path: str, time: float = 0.0, scale: bool = False, ev: np.ndarray | None = None,
+                      nut: np.ndarray | None = None) -> None:
+        """Tecplot `.plt` of the current state as the reference's wrtdat writes
+        it (dsem_amd.tecplot); scale=True divides out J (the movie files)."""
+        from . import tecplot
+        xg, jac = self.gauss_points()
+        tecplot.write_plt(path, time, xg, self.get_state(), jac, self.p, self.params["gamma"], self.params["mach"],
+                          scale, ev, nut)
+
     def gauss_points(self):
         xg = np.zeros((self.K, self.np, 3))
         jac = np.zeros((self.K, self.np))
