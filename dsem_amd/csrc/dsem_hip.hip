@@ -50,12 +50,14 @@ namespace {
 
 constexpr int NEQ = 5;
 constexpr int BLOCK = 256;
-// Fused volume kernel: one element per workgroup.  FP64 holds 149 KB of LDS
-// per element, so one workgroup fills a CU; 1024 work-items (16 waves, 127
-// VGPRs) keep 4 waves per SIMD in flight.  The FP32 modes (75 KB) fit two
-// workgroups of 512 per CU.
-template <int PM>
-constexpr int vblock() { return PM == 0 ? 1024 : 512; }
+// Fused volume kernel: one element per workgroup.  FP64 at p = 7, 8 holds
+// 105-151 KB of LDS per element, so one workgroup fills a CU, and its
+// matrix-core tiles (48, 63 per direction) need 1024 work-items (16 lanes
+// each); at p <= 6 every direction has at most 26 tiles, so 512 work-items
+// hold them all and two elements (<= 72 KB each) share a CU.  The FP32
+// modes (75 KB at p = 8) fit two workgroups of 512 per CU.
+template <int P, int PM>
+constexpr int vblock() { return (PM == 0 && P >= 7) ? 1024 : 512; }
 
 // operators for every order p = 1..8, indexed [p]
 __constant__ double c_D[9][90];   // D[k*nc + j]
@@ -450,13 +452,13 @@ __device__ __forceinline__ void volume_grid(const DevMesh &m, const DevPhys &ph,
                                             const double *sBar, const typename Ctr<PM>::T *sD,
                                             const typename Ctr<PM>::T *sB, const double *sDint, int e,
                                             const int *mort, const MfLane<P> &ml,
-                                            double (&acc)[(Sz<P>::NP + vblock<PM>() - 1) / vblock<PM>()][5])
+                                            double (&acc)[(Sz<P>::NP + vblock<P, PM>() - 1) / vblock<P, PM>()][5])
 {
     using S = Sz<P>;
     using CT = typename Ctr<PM>::T;
     using CA = typename Ctr<PM>::A;
     static_assert(!MF || PM == 0, "matrix-core contractions are FP64 only");
-    constexpr int VBLOCK = vblock<PM>();
+    constexpr int VBLOCK = vblock<P, PM>();
     static_assert(!MF || MfTiles<P>::NT <= VBLOCK / 16, "one tile per 16 lanes in one pass");
     constexpr int R = (S::NP + VBLOCK - 1) / VBLOCK;
     constexpr int KS = MfTiles<P>::KS;
@@ -694,7 +696,7 @@ __device__ __forceinline__ void volume_grid(const DevMesh &m, const DevPhys &ph,
 // Pointwise physics, the gradient metric combination inputs, the RK state
 // and all face kernels stay FP64.
 template <int P, int PM>
-__global__ __launch_bounds__(vblock<PM>()) void k_volume(const double *__restrict__ JQ, DevMesh m, DevPhys ph,
+__global__ __launch_bounds__((vblock<P, PM>())) void k_volume(const double *__restrict__ JQ, DevMesh m, DevPhys ph,
                                                     const double *__restrict__ QA, const double *__restrict__ muh,
                                                     const double *__restrict__ nut, double *__restrict__ FV,
                                                     double *__restrict__ Qtp, double *__restrict__ Slag)
@@ -702,7 +704,7 @@ __global__ __launch_bounds__(vblock<PM>()) void k_volume(const double *__restric
     using S = Sz<P>;
     using CT = typename Ctr<PM>::T;
     using CA = typename Ctr<PM>::A;
-    constexpr int VBLOCK = vblock<PM>();
+    constexpr int VBLOCK = vblock<P, PM>();
     constexpr int R = (S::NP + VBLOCK - 1) / VBLOCK;
     __shared__ CT sQ[NEQ * S::NP];            // physical Q at Gauss points (minus Qbar when mixed)
     __shared__ CT sG[15 * S::NP];             // dQ_v/dx_c at Gauss points, (c*5+v)
@@ -1694,11 +1696,11 @@ int run_rhs(dsem_ctx *c, double *JQ, const StageArgs &st, double *Qt_out, bool t
     double *slag = save_lag ? c->Slag : nullptr;
     TRY(launch(c, KID_VOLUME, [&] {
         if (c->params.precision == DSEM_PREC_FP32)
-            k_volume<P, 1><<<K, vblock<1>(), 0, s>>>(JQ, m, ph, c->QA, muh, nut, c->FV, c->Qtp, slag);
+            k_volume<P, 1><<<K, vblock<P, 1>(), 0, s>>>(JQ, m, ph, c->QA, muh, nut, c->FV, c->Qtp, slag);
         else if (c->params.precision == DSEM_PREC_BF16X3)
-            k_volume<P, 2><<<K, vblock<2>(), 0, s>>>(JQ, m, ph, c->QA, muh, nut, c->FV, c->Qtp, slag);
+            k_volume<P, 2><<<K, vblock<P, 2>(), 0, s>>>(JQ, m, ph, c->QA, muh, nut, c->FV, c->Qtp, slag);
         else
-            k_volume<P, 0><<<K, vblock<0>(), 0, s>>>(JQ, m, ph, c->QA, muh, nut, c->FV, c->Qtp, slag);
+            k_volume<P, 0><<<K, vblock<P, 0>(), 0, s>>>(JQ, m, ph, c->QA, muh, nut, c->FV, c->Qtp, slag);
     }));
     TRY(launch(c, KID_FACE_VISC, [&] { k_face_visc<P><<<face_blocks, BLOCK, 0, s>>>(c->FV, c->FI, m, ph, c->FT); }));
     if (make_traces)
