@@ -394,6 +394,14 @@ struct MfLane {
     bool wave_active;                 // some tile of this wave exists (uniform per wave)
 };
 
+// A value the compiler cannot see through (an empty asm that may change it):
+// keeps index arithmetic derived from it inside the loop that makes the copy.
+__device__ __forceinline__ int opaque(int x)
+{
+    asm volatile("" : "+v"(x));
+    return x;
+}
+
 template <int P, int ROWS = Sz<P>::NC>
 __device__ __forceinline__ MfLane<P, ROWS> mf_lane(int tid, const double *op)
 {
@@ -443,16 +451,23 @@ __device__ __forceinline__ double mf_line(const double *fld, const MfLane<P, ROW
 // MF (FP64 only): the interpolation of Q and the prolongation of the 15
 // gradients run on the matrix cores (mf_line), one point per lane; otherwise
 // each work-item forms its point's 9-term sums on the vector ALUs.
+// Flux of the lane's staggered point in the matrix-core form, held in
+// registers until every wave is past the previous divergence pass.
+struct LaneFlux {
+    double f[5];
+    int s;
+    bool store;
+};
+
 template <int P, int D, int PM, bool MF>
-__device__ __forceinline__ void volume_grid(const DevMesh &m, const DevPhys &ph, const double *__restrict__ QA,
+__device__ __forceinline__ LaneFlux volume_grid(const DevMesh &m, const DevPhys &ph, const double *__restrict__ QA,
                                             const double *__restrict__ muh, const double *__restrict__ nut,
                                             double *__restrict__ FV, double *__restrict__ Slag,
                                             const typename Ctr<PM>::T *sQ,
                                             const typename Ctr<PM>::T *sG, typename Ctr<PM>::T *sF,
                                             const double *sBar, const typename Ctr<PM>::T *sD,
                                             const typename Ctr<PM>::T *sB, const double *sDint, int e,
-                                            const int *mort, const MfLane<P> &ml,
-                                            double (&acc)[(Sz<P>::NP + vblock<P, PM>() - 1) / vblock<P, PM>()][5])
+                                            const int *e2m_e, const MfLane<P> &ml, int tid)
 {
     using S = Sz<P>;
     using CT = typename Ctr<PM>::T;
@@ -460,9 +475,7 @@ __device__ __forceinline__ void volume_grid(const DevMesh &m, const DevPhys &ph,
     static_assert(!MF || PM == 0, "matrix-core contractions are FP64 only");
     constexpr int VBLOCK = vblock<P, PM>();
     static_assert(!MF || MfTiles<P>::NT <= VBLOCK / 16, "one tile per 16 lanes in one pass");
-    constexpr int R = (S::NP + VBLOCK - 1) / VBLOCK;
     constexpr int KS = MfTiles<P>::KS;
-    const int tid = threadIdx.x;
     const int nsm = m.ms ? S::NS : 1, nfm = m.ms ? S::NF : 1;
     const double *gmd = (D == 0 ? m.gmet0 : (D == 1 ? m.gmet1 : m.gmet2)) + (size_t)e * 3 * nsm;
     // Matrix-core form: the lane's point is fixed, so its metric row and face
@@ -480,7 +493,7 @@ __device__ __forceinline__ void volume_grid(const DevMesh &m, const DevPhys &ph,
             if (nut) mf_nut = nut[((size_t)e * 3 + D) * S::NS + mf_s];
             if (muh) mf_muh = muh[((size_t)e * 3 + D) * S::NS + mf_s];
             if (ml.j == 0 || ml.j == S::NC - 1) {
-                const int mo = mort[face_of<P>(D, ml.j) - 1];
+                const int mo = e2m_e[face_of<P>(D, ml.j) - 1] >> 1;
                 const int q = a + S::NCG * b;
 #pragma unroll
                 for (int v = 0; v < 5; ++v) mf_qa[v] = QA[((size_t)mo * NEQ + v) * S::NF + q];
@@ -529,7 +542,7 @@ __device__ __forceinline__ void volume_grid(const DevMesh &m, const DevPhys &ph,
         const int s = MF ? mf_s : sidx<P>(D, jc, a, b);
         const bool face = (jc == 0 || jc == S::NC - 1);
         const int fc = face_of<P>(D, jc);
-        const int mo = mort[fc - 1];
+        const int mo = e2m_e[fc - 1] >> 1;
         double Qv[5];
         CT Bj[MF ? 1 : S::NCG];
         if constexpr (MF) {
@@ -648,14 +661,43 @@ __device__ __forceinline__ void volume_grid(const DevMesh &m, const DevPhys &ph,
             }
         }
     }
+    LaneFlux lf;
+#pragma unroll
+    for (int v = 0; v < 5; ++v) lf.f[v] = mf_f[v];
+    lf.s = mf_fs;
+    lf.store = mf_fstore;
+    return lf;
+}
+
+// Grid D's fluxes into sF.  The matrix-core form stores its lanes' fluxes
+// once every wave is past the previous divergence pass.  On return every
+// wave is past grid D's point pass (its reads of sQ and sG).
+template <int P, int D, int PM, bool MF>
+__device__ __forceinline__ void volume_publish(const LaneFlux &lf, typename Ctr<PM>::T *sF)
+{
+    using S = Sz<P>;
     if constexpr (MF) {
         if constexpr (D > 0) __syncthreads();   // every wave is past the previous divergence pass
-        if (mf_fstore) {
+        if (lf.store) {
 #pragma unroll
-            for (int v = 0; v < 5; ++v) sF[v * S::NS + mf_fs] = mf_f[v];
+            for (int v = 0; v < 5; ++v) sF[v * S::NS + lf.s] = lf.f[v];
         }
     }
     __syncthreads();   // fluxes of this grid complete
+}
+
+// Interior part of the divergence along grid D, with the grid's fluxes in
+// sF: acc -= sum_{interior j} D[k][j] F_j.
+template <int P, int D, int PM>
+__device__ __forceinline__ void volume_div(const typename Ctr<PM>::T *sF, const typename Ctr<PM>::T *sD,
+                                           const double *sBar, const double *sDint, int tid,
+                                           double (&acc)[(Sz<P>::NP + vblock<P, PM>() - 1) / vblock<P, PM>()][5])
+{
+    using S = Sz<P>;
+    using CT = typename Ctr<PM>::T;
+    using CA = typename Ctr<PM>::A;
+    constexpr int VBLOCK = vblock<P, PM>();
+    constexpr int R = (S::NP + VBLOCK - 1) / VBLOCK;
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         const int pt = tid + r * VBLOCK;
@@ -676,6 +718,115 @@ __device__ __forceinline__ void volume_grid(const DevMesh &m, const DevPhys &ph,
                 else acc[r][v] -= (double)t + sBar[5 + D * 5 + v] * sDint[k];
             }
         }
+    }
+}
+
+// Direct global->LDS copy (global_load_lds_dword): lane l of a wave moves one
+// dword from its own address src to the LDS byte address M0 + 4 l, through no
+// register.  Written as inline asm on purpose: the compiler's own form makes
+// every later LDS read wait for all such copies (it cannot tell the
+// destinations apart), which would serialize the copies with the work they
+// are meant to overlap.  The caller waits for them itself (lds_copies_done)
+// before the barrier that publishes them.  lds_wave_base must be the same for
+// every lane of the wave.
+typedef __attribute__((address_space(3))) void *LdsPtr;
+
+__device__ __forceinline__ void lds_dword(const void *src, void *lds_wave_base)
+{
+    const unsigned l = __builtin_amdgcn_readfirstlane((unsigned)(size_t)(LdsPtr)lds_wave_base);
+    // M0 is reserved by the compiler, which sets it right before each
+    // instruction of its own that reads it (the volume kernel has none)
+    asm volatile("s_mov_b32 m0, %1\n\ts_nop 0\n\tglobal_load_lds_dword %0, off" : : "v"(src), "s"(l) : "memory");
+}
+
+// s_waitcnt immediate of gfx9 for vmcnt(0) alone (expcnt 7, lgkmcnt 15).
+constexpr int VMCNT0 = 0x0F70;
+
+// Every copy this wave issued has landed in LDS.
+__device__ __forceinline__ void lds_copies_done()
+{
+    asm volatile("s_waitcnt vmcnt(0)" : : : "memory");
+}
+
+// The next element's volume inputs, copied straight into LDS while the
+// current element finishes: raw J*Q into sQ ([v][pt]), 1/J at every Gauss
+// point into sJ, and the six faces' averages (face f+1 at [f][v][q],
+// gathered by the mortars of the element's e2m entries e2m_e) into sA.
+// Every wave copies 64-dword chunks w, w + NW, ...; each wave waits for its
+// copies (lds_copies_done) before the barrier that publishes them.
+template <int P>
+__device__ __forceinline__ void volume_inputs_lds(const double *__restrict__ JQ, const double *__restrict__ QA,
+                                                  const DevMesh &m, int e, const int *e2m_e, int tid,
+                                                  double *sQ, double *sJ, double *sA)
+{
+    using S = Sz<P>;
+    constexpr int NW = vblock<P, 0>() / 64;
+    constexpr int NQV = NEQ * S::NP, NAV = 6 * NEQ * S::NF;
+    constexpr int DQ = 2 * NQV, DJ = 2 * S::NP, DA = 2 * NAV;   // dwords
+    constexpr int CQ = (DQ + 63) / 64, CJ = (DJ + 63) / 64, CA = (DA + 63) / 64;   // 64-dword chunks
+    const int lane = tid % 64;
+    const int w = __builtin_amdgcn_readfirstlane(tid / 64);
+    // mortars first, before any copy is in flight
+    int mo6[6];
+#pragma unroll
+    for (int g = 0; g < 6; ++g) mo6[g] = e2m_e[g] >> 1;
+    const char *q = (const char *)(JQ + (size_t)e * NQV);
+    const char *jac = (const char *)(m.jacob + (size_t)e * (m.ms ? S::NP : 1));
+#pragma unroll
+    for (int u = 0; u < (CQ + NW - 1) / NW; ++u) {
+        const int c = w + u * NW, d = c * 64 + lane;
+        if (d < DQ) lds_dword(q + 4 * d, (char *)sQ + 256 * c);
+    }
+#pragma unroll
+    for (int u = 0; u < (CJ + NW - 1) / NW; ++u) {
+        const int c = w + u * NW, d = c * 64 + lane;
+        if (d < DJ) lds_dword(jac + 8 * ((d >> 1) * m.ms) + 4 * (d & 1), (char *)sJ + 256 * c);
+    }
+#pragma unroll
+    for (int u = 0; u < (CA + NW - 1) / NW; ++u) {
+        const int c = w + u * NW, d = c * 64 + lane;
+        const int i = d >> 1, f = i / (NEQ * S::NF), vq = i % (NEQ * S::NF);
+        int mo = mo6[0];
+#pragma unroll
+        for (int g = 1; g < 6; ++g) mo = f == g ? mo6[g] : mo;
+        if (d < DA) lds_dword((const char *)(QA + (size_t)mo * NEQ * S::NF + vq) + 4 * (d & 1), (char *)sA + 256 * c);
+    }
+}
+
+// Global loads of one element's state and 1/J and of its six faces'
+// averages, all issued before any use.
+template <int P, int PM>
+__device__ __forceinline__ void volume_inputs(const double *__restrict__ JQ, const double *__restrict__ QA,
+                                              const DevMesh &m, int e, int tid,
+                                              double (&qv)[(NEQ * Sz<P>::NP + vblock<P, PM>() - 1) / vblock<P, PM>()],
+                                              double (&jv)[(NEQ * Sz<P>::NP + vblock<P, PM>() - 1) / vblock<P, PM>()],
+                                              double (&av)[(6 * NEQ * Sz<P>::NF + vblock<P, PM>() - 1) / vblock<P, PM>()])
+{
+    using S = Sz<P>;
+    constexpr int VBLOCK = vblock<P, PM>();
+    constexpr int NQV = NEQ * S::NP, NAV = 6 * NEQ * S::NF;
+    constexpr int NQI = (NQV + VBLOCK - 1) / VBLOCK, NAI = (NAV + VBLOCK - 1) / VBLOCK;
+    const double *q = JQ + (size_t)e * NEQ * S::NP;
+    const double *jac = m.jacob + (size_t)e * (m.ms ? S::NP : 1);
+    const int *e2m = m.e2m + (size_t)e * 6;
+    int mort[6];
+#pragma unroll
+    for (int f = 0; f < 6; ++f) mort[f] = e2m[f] >> 1;
+#pragma unroll
+    for (int u = 0; u < NQI; ++u) {
+        const int i = tid + u * VBLOCK;
+        qv[u] = i < NQV ? q[i] : 0.0;
+        jv[u] = i < NQV ? jac[(i % S::NP) * m.ms] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < NAI; ++u) {
+        const int i = tid + u * VBLOCK;
+        const int f = i / (NEQ * S::NF), vq = i % (NEQ * S::NF);
+        // mortar of face f+1 by selects (a dynamic index into mort[] would go to scratch)
+        int mo = mort[0];
+#pragma unroll
+        for (int g = 1; g < 6; ++g) mo = f == g ? mort[g] : mo;
+        av[u] = i < NAV ? QA[(size_t)mo * NEQ * S::NF + vq] : 0.0;
     }
 }
 
@@ -717,215 +868,272 @@ __global__ __launch_bounds__((vblock<P, PM>())) void k_volume(const double *__re
     constexpr int OM = 0, OD = S::NCG * S::NCG, OB = OD + S::NCG * S::NC, NOP = OB + S::NC * S::NCG;
     __shared__ CT sOp[NOP];
     __shared__ double sDint[S::NCG];          // mixed modes: sum over interior j of D[k][j]
-    const int e = blockIdx.x;
-    const int tid = threadIdx.x;
-    const double *jac = m.jacob + (size_t)e * (m.ms ? S::NP : 1);
-    const int *e2m = m.e2m + (size_t)e * 6;
-    int mort[6];
-#pragma unroll
-    for (int f = 0; f < 6; ++f) mort[f] = e2m[f] >> 1;
-    const int npm = m.ms ? S::NP : 1;
-
-    // ---- prologue.  Every global load of the element's inputs is issued
-    // before the first wait: one exposed memory latency per element instead
-    // of one per loop trip (one workgroup per CU leaves nothing to hide it).
+    const int tid0 = threadIdx.x;
+    constexpr bool MF = (PM == 0);
     constexpr int NQV = NEQ * S::NP, NAV = 6 * NEQ * S::NF;
     constexpr int NQI = (NQV + VBLOCK - 1) / VBLOCK, NAI = (NAV + VBLOCK - 1) / VBLOCK;
     constexpr int NOI = (NOP + VBLOCK - 1) / VBLOCK;
-    double qv[NQI], jv[NQI], av[NAI];   // Q and 1/J kept apart: a product here would wait for both loads
-    CT ov[NOI];
-    CA gpre[R][9];   // metric rows of this work-item's Gauss points (gradient phase)
-    {
-        const double *q = JQ + (size_t)e * NEQ * S::NP;
-#pragma unroll
-        for (int u = 0; u < NQI; ++u) {
-            const int i = tid + u * VBLOCK;
-            qv[u] = i < NQV ? q[i] : 0.0;
-            jv[u] = i < NQV ? jac[(i % S::NP) * m.ms] : 0.0;
-        }
-#pragma unroll
-        for (int u = 0; u < NAI; ++u) {
-            const int i = tid + u * VBLOCK;
-            const int f = i / (NEQ * S::NF), vq = i % (NEQ * S::NF);
-            // mortar of face f+1 by selects (a dynamic index into mort[] would go to scratch)
-            int mo = mort[0];
-#pragma unroll
-            for (int g = 1; g < 6; ++g) mo = f == g ? mort[g] : mo;
-            av[u] = i < NAV ? QA[(size_t)mo * NEQ * S::NF + vq] : 0.0;
-        }
-        const double *gg = m.gmetg + (size_t)e * 9 * npm;
-#pragma unroll
-        for (int r = 0; r < R; ++r) {
-            const int pt = tid + r * VBLOCK;
-#pragma unroll
-            for (int i = 0; i < 9; ++i) gpre[r][i] = pt < S::NP ? (CA)gg[i * npm + pt * m.ms] : (CA)0;
-        }
-#pragma unroll
-        for (int u = 0; u < NOI; ++u) {
-            const int i = tid + u * VBLOCK;
-            // one unconditional load from a selected address (branches here made the compiler wait)
-            const int ic = i < NOP ? i : NOP - 1;
-            const CT *src;
-            if constexpr (PM == 0) src = ic < OD ? &c_M[P][ic - OM] : (ic < OB ? &c_D[P][ic - OD] : &c_B[P][ic - OB]);
-            else src = ic < OD ? &c_Mf[P][ic - OM] : (ic < OB ? &c_Df[P][ic - OD] : &c_Bf[P][ic - OB]);
-            ov[u] = *src;
-        }
-    }
-    constexpr bool MF = (PM == 0);
-    const MfLane<P> ml = mf_lane<P>(tid, c_B[P]);
-    const MfLane<P, S::NCG> mg = mf_lane<P, S::NCG>(tid, c_M[P]);   // gradient tiles (FP64 form)
-    if constexpr (PM != 0) {
-        // Qbar = Q at the central Gauss point, Fbar_d = Euler flux of Qbar
-        // with the metric row at staggered point 0 of grid d
-        constexpr int pc = (S::NCG / 2) * (1 + S::NCG + S::NCG * S::NCG);
-        const int nsm = m.ms ? S::NS : 1;
-        double qc = 0.0, gm[3] = {0.0, 0.0, 0.0};
-        if (tid < 5) qc = JQ[((size_t)e * NEQ + tid) * S::NP + pc] * jac[pc * m.ms];
-        if (tid < 3) {
-            const double *gmd = (tid == 0 ? m.gmet0 : (tid == 1 ? m.gmet1 : m.gmet2)) + (size_t)e * 3 * nsm;
-#pragma unroll
-            for (int i = 0; i < 3; ++i) gm[i] = gmd[i * nsm];
-        }
-        if (tid < S::NCG) sDint[tid] = c_Dint[P][tid];
-        if (tid < 5) sBar[tid] = qc;
-        __syncthreads();
-        if (tid < 3) {
-            double qb[5], fb[5];
-#pragma unroll
-            for (int v = 0; v < 5; ++v) qb[v] = sBar[v];
-            const Prim pb = primitives(ph, qb);
-            euler_from_primitive(pb, qb, gm[0], gm[1], gm[2], fb);
-#pragma unroll
-            for (int v = 0; v < 5; ++v) sBar[5 + tid * 5 + v] = fb[v];
-        }
-    }
-#pragma unroll
-    for (int u = 0; u < NQI; ++u) {
-        const int i = tid + u * VBLOCK;
-        if (i < NQV) {
-            const double qp = qv[u] * jv[u];
-            if constexpr (PM == 0) sQ[i] = qp; else sQ[i] = (CT)(qp - sBar[i / S::NP]);
-        }
-    }
-    // face averages of the six faces (face f+1 at [f][v][q]), minus Qbar when mixed
-    CT *sQA = sF;
-#pragma unroll
-    for (int u = 0; u < NAI; ++u) {
-        const int i = tid + u * VBLOCK;
-        if (i < NAV) {
-            if constexpr (PM == 0) sQA[i] = av[u]; else sQA[i] = (CT)(av[u] - sBar[(i % (NEQ * S::NF)) / S::NF]);
-        }
-    }
+    static_assert(NAV <= 15 * S::NP, "the face averages of the next element fit in sG");
+    // operator matrices, stored once per workgroup (the first element's
+    // barrier after its staging publishes them)
 #pragma unroll
     for (int u = 0; u < NOI; ++u) {
-        const int i = tid + u * VBLOCK;
-        if (i < NOP) sOp[i] = ov[u];
+        const int i = tid0 + u * VBLOCK;
+        if (i < NOP) {
+            const CT *src;
+            if constexpr (PM == 0) src = i < OD ? &c_M[P][i - OM] : (i < OB ? &c_D[P][i - OD] : &c_B[P][i - OB]);
+            else src = i < OD ? &c_Mf[P][i - OM] : (i < OB ? &c_Df[P][i - OD] : &c_Bf[P][i - OB]);
+            sOp[i] = *src;
+        }
     }
-    __syncthreads();
+    // Persistent workgroup: elements blockIdx.x, + gridDim.x, ...  At 1024
+    // work-items (one workgroup per CU, nothing else to hide a memory wait)
+    // the FP64 form copies the next element's state, 1/J and face averages
+    // straight into sQ and sG (both free once the last grid's point pass is
+    // done) during the last divergence pass, so that only a workgroup's first
+    // element waits for its prologue.
+    constexpr bool PRE = MF && VBLOCK == 1024;
+    static_assert(!PRE || NAV + S::NP <= 15 * S::NP, "the next element's 1/J and face averages fit in sG");
+    // PRE: e2m entries of this element and of the next, in alternating slots
+    __shared__ int sE2m[2][6];
+    const int npm = m.ms ? S::NP : 1;
+    if constexpr (PRE) {
+        // the workgroup's first element: its e2m entries and those of the
+        // next, then its inputs copied into LDS like every later element's
+        const int e0 = blockIdx.x, e1 = min(e0 + (int)gridDim.x, m.K - 1);
+        if (tid0 < 6) sE2m[0][tid0] = m.e2m[(size_t)e0 * 6 + tid0];
+        else if (tid0 < 12) sE2m[1][tid0 - 6] = m.e2m[(size_t)e1 * 6 + tid0 - 6];
+        __syncthreads();
+        volume_inputs_lds<P>(JQ, QA, m, e0, sE2m[0], tid0, sQ, sG + NAV, sG);
+    }
+    // Without PRE the body runs once (grid = elements) and there is no loop.
+    int slot = 0;
+    int e = blockIdx.x;
+#pragma unroll 1
+    while (true) {
+        const bool first = (e == blockIdx.x);
+        // The work-item index passes through an empty asm once per element:
+        // left visible, every index the element's passes derive from it (and
+        // the matrix-core operand fragments) is hoisted out of the loop and
+        // held in registers for its whole length (past 128 VGPRs, and into
+        // scratch, at 1024 work-items).
+        const int tid = PRE ? opaque(tid0) : tid0;
+        const MfLane<P> ml = mf_lane<P>(tid, c_B[P]);
+        const MfLane<P, S::NCG> mg = mf_lane<P, S::NCG>(tid, c_M[P]);   // gradient tiles (FP64 form)
+        CA gpre[R][9];
+        const double *jac = m.jacob + (size_t)e * (m.ms ? S::NP : 1);
+        const int nxt = e + gridDim.x;
+        int e2r[6];
+        const int *e2m_e = e2r;
+        if constexpr (PRE) {
+            e2m_e = sE2m[slot];
+        } else {
+#pragma unroll
+            for (int f = 0; f < 6; ++f) e2r[f] = m.e2m[(size_t)e * 6 + f];
+        }
 
-    // ---- gradients: w_d = (D_int B) Q + D(:,0) QA_lo + D(:,nc-1) QA_hi, then metric combine
-    if constexpr (MF) {
-        // Matrix-core form.  Direction by direction, the tile lane of each
-        // Gauss point writes w_d (5 fields) into slots d*5+v of sG; one
-        // direction's tiles cover every point once, so the three directions
-        // need no barrier between them.  Then each work-item combines the 15
-        // w values of its point with the metric rows, in place, in the same
-        // order as the vector form: g0 w0, then + g1 w1, then + g2 w2.
-        constexpr int KSG = MfTiles<P, S::NCG>::KS;
-        static_assert(MfTiles<P, S::NCG>::NT <= VBLOCK / 16, "one gradient tile per 16 lanes in one pass");
-        if (mg.wave_active) {
-            const int a = mg.line % S::NCG, b = mg.line / S::NCG;   // mg.line = a + ncg b
+        // ---- prologue.  Every global load of the element's inputs is issued
+        // before the first wait: one exposed memory latency per element
+        // instead of one per loop trip.
+        double qv[NQI], jv[NQI], av[NAI];
+        if constexpr (!PRE) volume_inputs<P, PM>(JQ, QA, m, e, tid, qv, jv, av);
+        // the previous element is done with sQ, sG, sF; PRE: this element's
+        // copies into LDS have landed
+        if constexpr (PRE) lds_copies_done();
+        if (PRE || !first) __syncthreads();
+        {
+            // metric rows of this work-item's Gauss points, first used after
+            // the gradient contractions
+            const double *gg = m.gmetg + (size_t)e * 9 * npm;
 #pragma unroll
-            for (int d = 0; d < 3; ++d) {
-                int off[KSG];
+            for (int r = 0; r < R; ++r) {
+                const int pt = tid + r * VBLOCK;
 #pragma unroll
-                for (int ks = 0; ks < KSG; ++ks) {
-                    const int k = 4 * ks + mg.r;
-                    off[ks] = gidx<P>(d, k < S::NCG ? k : S::NCG - 1, a, b);
+                for (int i = 0; i < 9; ++i) gpre[r][i] = pt < S::NP ? (CA)gg[i * npm + pt * m.ms] : (CA)0;
+            }
+        }
+        if constexpr (PM != 0) {
+            // Qbar = Q at the central Gauss point, Fbar_d = Euler flux of Qbar
+            // with the metric row at staggered point 0 of grid d
+            constexpr int pc = (S::NCG / 2) * (1 + S::NCG + S::NCG * S::NCG);
+            const int nsm = m.ms ? S::NS : 1;
+            double qc = 0.0, gm[3] = {0.0, 0.0, 0.0};
+            if (tid < 5) qc = JQ[((size_t)e * NEQ + tid) * S::NP + pc] * jac[pc * m.ms];
+            if (tid < 3) {
+                const double *gmd = (tid == 0 ? m.gmet0 : (tid == 1 ? m.gmet1 : m.gmet2)) + (size_t)e * 3 * nsm;
+#pragma unroll
+                for (int i = 0; i < 3; ++i) gm[i] = gmd[i * nsm];
+            }
+            if (tid < S::NCG) sDint[tid] = c_Dint[P][tid];
+            if (tid < 5) sBar[tid] = qc;
+            __syncthreads();
+            if (tid < 3) {
+                double qb[5], fb[5];
+#pragma unroll
+                for (int v = 0; v < 5; ++v) qb[v] = sBar[v];
+                const Prim pb = primitives(ph, qb);
+                euler_from_primitive(pb, qb, gm[0], gm[1], gm[2], fb);
+#pragma unroll
+                for (int v = 0; v < 5; ++v) sBar[5 + tid * 5 + v] = fb[v];
+            }
+        }
+
+        CT *sQA = sF;   // face averages of the six faces (face f+1 at [f][v][q]), minus Qbar when mixed
+        if constexpr (PRE) {
+            // sQ holds this element's raw J*Q, sG its face averages and (from
+            // NAV on) 1/J: Q = (J*Q) (1/J) in place
+            for (int i = tid; i < NQV; i += VBLOCK) sQ[i] = sQ[i] * sG[NAV + i % S::NP];
+            for (int i = tid; i < NAV; i += VBLOCK) sQA[i] = sG[i];
+        } else {
+#pragma unroll
+            for (int u = 0; u < NQI; ++u) {
+                const int i = tid + u * VBLOCK;
+                if (i < NQV) {
+                    const double qp = qv[u] * jv[u];
+                    if constexpr (PM == 0) sQ[i] = qp; else sQ[i] = (CT)(qp - sBar[i / S::NP]);
                 }
-                double w[5];
+            }
 #pragma unroll
-                for (int v = 0; v < 5; ++v) w[v] = mf_line<P, S::NCG>(sQ + v * S::NP, mg, off);
-                if (mg.valid) {
-                    const int k = mg.j;
-                    const double d0 = sOp[OD + k * S::NC], d1 = sOp[OD + k * S::NC + S::NC - 1];
-                    const double *qa0 = sQA + (face_of<P>(d, 0) - 1) * NEQ * S::NF + mg.line;
-                    const double *qa1 = sQA + (face_of<P>(d, 1) - 1) * NEQ * S::NF + mg.line;
-                    const int pt = gidx<P>(d, k, a, b);
-#pragma unroll
-                    for (int v = 0; v < 5; ++v)
-                        sG[(d * 5 + v) * S::NP + pt] = fma(d1, qa1[v * S::NF], fma(d0, qa0[v * S::NF], w[v]));
+            for (int u = 0; u < NAI; ++u) {
+                const int i = tid + u * VBLOCK;
+                if (i < NAV) {
+                    if constexpr (PM == 0) sQA[i] = av[u]; else sQA[i] = (CT)(av[u] - sBar[(i % (NEQ * S::NF)) / S::NF]);
                 }
             }
         }
         __syncthreads();
-#pragma unroll
-        for (int r = 0; r < R; ++r) {
-            const int pt = tid + r * VBLOCK;
-            if (pt >= S::NP) break;
-            double w[15];
-#pragma unroll
-            for (int i = 0; i < 15; ++i) w[i] = sG[i * S::NP + pt];
-            const CA *g = gpre[r];
-#pragma unroll
-            for (int c = 0; c < 3; ++c)
-#pragma unroll
-                for (int v = 0; v < 5; ++v)
-                    sG[(c * 5 + v) * S::NP + pt] = fma(g[6 + c], w[10 + v], fma(g[3 + c], w[5 + v], g[c] * w[v]));
-        }
-    } else {
-#pragma unroll
-        for (int r = 0; r < R; ++r) {
-            const int pt = tid + r * VBLOCK;
-            if (pt >= S::NP) break;
-            const int n = pt % S::NCG, mi = (pt / S::NCG) % S::NCG, l = pt / (S::NCG * S::NCG);
-            CA out[3][5];
-#pragma unroll
-            for (int d = 0; d < 3; ++d) {
-                const int k = d == 0 ? n : (d == 1 ? mi : l);
-                const int a = d == 0 ? mi : n;
-                const int b = d == 2 ? mi : l;
-                const int q = a + S::NCG * b;
-                CT Mk[S::NCG];
-#pragma unroll
-                for (int k2 = 0; k2 < S::NCG; ++k2) Mk[k2] = sOp[OM + k * S::NCG + k2];
-                const CT d0 = sOp[OD + k * S::NC], d1 = sOp[OD + k * S::NC + S::NC - 1];
-                const CT *qa0 = sQA + (face_of<P>(d, 0) - 1) * NEQ * S::NF + q;
-                const CT *qa1 = sQA + (face_of<P>(d, 1) - 1) * NEQ * S::NF + q;
-                const CA *gd = gpre[r] + d * 3;
-#pragma unroll
-                for (int v = 0; v < 5; ++v) {
-                    CA t = 0;
-#pragma unroll
-                    for (int k2 = 0; k2 < S::NCG; ++k2) t = mac<PM>(Mk[k2], sQ[v * S::NP + gidx<P>(d, k2, a, b)], t);
-                    t = mac<PM>(d0, qa0[v * S::NF], t);
-                    t = mac<PM>(d1, qa1[v * S::NF], t);
-#pragma unroll
-                    for (int c = 0; c < 3; ++c) out[c][v] = d == 0 ? gd[c] * t : fma(gd[c], t, out[c][v]);
-                }
-                __builtin_amdgcn_sched_barrier(0);
-            }
-#pragma unroll
-            for (int c = 0; c < 3; ++c)
-#pragma unroll
-                for (int v = 0; v < 5; ++v) sG[(c * 5 + v) * S::NP + pt] = (CT)out[c][v];
-        }
-    }
 
-    double acc[R][5];
+        // ---- gradients: w_d = (D_int B) Q + D(:,0) QA_lo + D(:,nc-1) QA_hi, then metric combine
+        if constexpr (MF) {
+            // Matrix-core form.  Direction by direction, the tile lane of each
+            // Gauss point writes w_d (5 fields) into slots d*5+v of sG; one
+            // direction's tiles cover every point once, so the three directions
+            // need no barrier between them.  Then each work-item combines the 15
+            // w values of its point with the metric rows, in place, in the same
+            // order as the vector form: g0 w0, then + g1 w1, then + g2 w2.
+            constexpr int KSG = MfTiles<P, S::NCG>::KS;
+            static_assert(MfTiles<P, S::NCG>::NT <= VBLOCK / 16, "one gradient tile per 16 lanes in one pass");
+            if (mg.wave_active) {
+                const int a = mg.line % S::NCG, b = mg.line / S::NCG;   // mg.line = a + ncg b
 #pragma unroll
-    for (int r = 0; r < R; ++r)
+                for (int d = 0; d < 3; ++d) {
+                    int off[KSG];
 #pragma unroll
-        for (int v = 0; v < 5; ++v) acc[r][v] = 0.0;
-    volume_grid<P, 0, PM, MF>(m, ph, QA, muh, nut, FV, Slag, sQ, sG, sF, sBar, sOp + OD, sOp + OB, sDint, e, mort, ml, acc);
-    volume_grid<P, 1, PM, MF>(m, ph, QA, muh, nut, FV, Slag, sQ, sG, sF, sBar, sOp + OD, sOp + OB, sDint, e, mort, ml, acc);
-    volume_grid<P, 2, PM, MF>(m, ph, QA, muh, nut, FV, Slag, sQ, sG, sF, sBar, sOp + OD, sOp + OB, sDint, e, mort, ml, acc);
+                    for (int ks = 0; ks < KSG; ++ks) {
+                        const int k = 4 * ks + mg.r;
+                        off[ks] = gidx<P>(d, k < S::NCG ? k : S::NCG - 1, a, b);
+                    }
+                    double w[5];
 #pragma unroll
-    for (int r = 0; r < R; ++r) {
-        const int pt = tid + r * VBLOCK;
-        if (pt < S::NP)
+                    for (int v = 0; v < 5; ++v) w[v] = mf_line<P, S::NCG>(sQ + v * S::NP, mg, off);
+                    if (mg.valid) {
+                        const int k = mg.j;
+                        const double d0 = sOp[OD + k * S::NC], d1 = sOp[OD + k * S::NC + S::NC - 1];
+                        const double *qa0 = sQA + (face_of<P>(d, 0) - 1) * NEQ * S::NF + mg.line;
+                        const double *qa1 = sQA + (face_of<P>(d, 1) - 1) * NEQ * S::NF + mg.line;
+                        const int pt = gidx<P>(d, k, a, b);
 #pragma unroll
-            for (int v = 0; v < 5; ++v) Qtp[((size_t)e * NEQ + v) * S::NP + pt] = acc[r][v];
+                        for (int v = 0; v < 5; ++v)
+                            sG[(d * 5 + v) * S::NP + pt] = fma(d1, qa1[v * S::NF], fma(d0, qa0[v * S::NF], w[v]));
+                    }
+                }
+            }
+            __syncthreads();
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const int pt = tid + r * VBLOCK;
+                if (pt >= S::NP) break;
+                double w[15];
+#pragma unroll
+                for (int i = 0; i < 15; ++i) w[i] = sG[i * S::NP + pt];
+                const CA *g = gpre[r];
+#pragma unroll
+                for (int c = 0; c < 3; ++c)
+#pragma unroll
+                    for (int v = 0; v < 5; ++v)
+                        sG[(c * 5 + v) * S::NP + pt] = fma(g[6 + c], w[10 + v], fma(g[3 + c], w[5 + v], g[c] * w[v]));
+            }
+        } else {
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const int pt = tid + r * VBLOCK;
+                if (pt >= S::NP) break;
+                const int n = pt % S::NCG, mi = (pt / S::NCG) % S::NCG, l = pt / (S::NCG * S::NCG);
+                CA out[3][5];
+#pragma unroll
+                for (int d = 0; d < 3; ++d) {
+                    const int k = d == 0 ? n : (d == 1 ? mi : l);
+                    const int a = d == 0 ? mi : n;
+                    const int b = d == 2 ? mi : l;
+                    const int q = a + S::NCG * b;
+                    CT Mk[S::NCG];
+#pragma unroll
+                    for (int k2 = 0; k2 < S::NCG; ++k2) Mk[k2] = sOp[OM + k * S::NCG + k2];
+                    const CT d0 = sOp[OD + k * S::NC], d1 = sOp[OD + k * S::NC + S::NC - 1];
+                    const CT *qa0 = sQA + (face_of<P>(d, 0) - 1) * NEQ * S::NF + q;
+                    const CT *qa1 = sQA + (face_of<P>(d, 1) - 1) * NEQ * S::NF + q;
+                    const CA *gd = gpre[r] + d * 3;
+#pragma unroll
+                    for (int v = 0; v < 5; ++v) {
+                        CA t = 0;
+#pragma unroll
+                        for (int k2 = 0; k2 < S::NCG; ++k2) t = mac<PM>(Mk[k2], sQ[v * S::NP + gidx<P>(d, k2, a, b)], t);
+                        t = mac<PM>(d0, qa0[v * S::NF], t);
+                        t = mac<PM>(d1, qa1[v * S::NF], t);
+#pragma unroll
+                        for (int c = 0; c < 3; ++c) out[c][v] = d == 0 ? gd[c] * t : fma(gd[c], t, out[c][v]);
+                    }
+                    __builtin_amdgcn_sched_barrier(0);
+                }
+#pragma unroll
+                for (int c = 0; c < 3; ++c)
+#pragma unroll
+                    for (int v = 0; v < 5; ++v) sG[(c * 5 + v) * S::NP + pt] = (CT)out[c][v];
+            }
+        }
+
+
+        double acc[R][5];
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+#pragma unroll
+            for (int v = 0; v < 5; ++v) acc[r][v] = 0.0;
+        LaneFlux lf;
+        lf = volume_grid<P, 0, PM, MF>(m, ph, QA, muh, nut, FV, Slag, sQ, sG, sF, sBar, sOp + OD, sOp + OB, sDint, e, e2m_e, ml, tid);
+        volume_publish<P, 0, PM, MF>(lf, sF);
+        volume_div<P, 0, PM>(sF, sOp + OD, sBar, sDint, tid, acc);
+        lf = volume_grid<P, 1, PM, MF>(m, ph, QA, muh, nut, FV, Slag, sQ, sG, sF, sBar, sOp + OD, sOp + OB, sDint, e, e2m_e, ml, tid);
+        volume_publish<P, 1, PM, MF>(lf, sF);
+        volume_div<P, 1, PM>(sF, sOp + OD, sBar, sDint, tid, acc);
+        lf = volume_grid<P, 2, PM, MF>(m, ph, QA, muh, nut, FV, Slag, sQ, sG, sF, sBar, sOp + OD, sOp + OB, sDint, e, e2m_e, ml, tid);
+        volume_publish<P, 2, PM, MF>(lf, sF);
+        if constexpr (PRE) {
+            // wait for this element's face-trace stores first, in a form the
+            // compiler accounts for (on every path): a wait it placed later
+            // for them would also wait for every copy issued after them
+            __builtin_amdgcn_s_waitcnt(VMCNT0);
+            if (nxt < m.K) {
+                // every wave is past this element's point passes: the next
+                // element's inputs go into sQ and sG, and the e2m entries of
+                // the element after it into the slot this element is done with
+                const int *e2m_n = sE2m[slot ^ 1];
+                volume_inputs_lds<P>(JQ, QA, m, nxt, e2m_n, tid, sQ, sG + NAV, sG);
+                const int n2 = min(nxt + (int)gridDim.x, m.K - 1);
+                if (tid < 6) lds_dword(m.e2m + (size_t)n2 * 6 + tid, sE2m[slot]);
+            }
+        }
+        volume_div<P, 2, PM>(sF, sOp + OD, sBar, sDint, tid, acc);
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int pt = tid + r * VBLOCK;
+            if (pt < S::NP)
+#pragma unroll
+                for (int v = 0; v < 5; ++v) Qtp[((size_t)e * NEQ + v) * S::NP + pt] = acc[r][v];
+        }
+        if constexpr (!PRE) break;
+        e += gridDim.x;
+        slot ^= 1;
+        if (e >= m.K) break;
     }
 }
 
@@ -1565,6 +1773,7 @@ struct dsem_ctx {
     bool have_lag = false;
     bool keep_lag = false;                     // save the last stage's staggered states without EV
     bool traces_cur = false;                   // T holds the face traces of the current JQ
+    int vol_grid[3] = {0, 0, 0};               // resident volume-kernel workgroups per precision mode
     bool les = false;                          // static Smagorinsky model
     double les_cs = 0.0, les_numax = 0.0;
     double *nut = nullptr, *nut_g = nullptr;   // eddy viscosity on the staggered grids / Gauss points
@@ -1673,6 +1882,29 @@ int launch(dsem_ctx *c, int kid, F &&fn)
         if (rc_) return rc_;   \
     } while (0)
 
+// Workgroups of the volume kernel.  The FP64 kernels of 1024 work-items
+// (p >= 7) are persistent: as many workgroups as are resident at once, each
+// looping over elements with a stride of the grid (never more than the
+// elements; queried once per context).  The others take one element each.
+template <int P, int PM>
+int volume_grid_size(dsem_ctx *c, int &grid)
+{
+    if (!(PM == 0 && vblock<P, PM>() == 1024)) {
+        grid = c->K;
+        return DSEM_OK;
+    }
+    int &g = c->vol_grid[PM];
+    if (g == 0) {
+        int per_cu = 0, ncu = 0;
+        HIP_TRY(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_volume<P, PM>, vblock<P, PM>(), 0));
+        HIP_TRY(c, hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->device));
+        if (per_cu < 1 || ncu < 1) return set_err(c, DSEM_ERR_HIP, "volume kernel: no resident work-group fits");
+        g = per_cu * ncu;
+    }
+    grid = g < c->K ? g : c->K;
+    return DSEM_OK;
+}
+
 // One RHS evaluation on state JQ; the final kernel either writes Qt or
 // performs an RK register update (st.rk).
 // traces_ready: T already holds the face traces of JQ (computed by the
@@ -1694,13 +1926,17 @@ int run_rhs(dsem_ctx *c, double *JQ, const StageArgs &st, double *Qt_out, bool t
     if (!traces_ready) TRY(launch(c, KID_TRACES, [&] { k_traces<P><<<K, BLOCK, 0, s>>>(JQ, m, c->T); }));
     TRY(launch(c, KID_FACE_INV, [&] { k_face_inv<P><<<face_blocks, BLOCK, 0, s>>>(c->T, m, ph, c->FI, c->QA); }));
     double *slag = save_lag ? c->Slag : nullptr;
+    int gv0 = 0, gv1 = 0, gv2 = 0;
+    if (c->params.precision == DSEM_PREC_FP32) TRY((volume_grid_size<P, 1>(c, gv1)));
+    else if (c->params.precision == DSEM_PREC_BF16X3) TRY((volume_grid_size<P, 2>(c, gv2)));
+    else TRY((volume_grid_size<P, 0>(c, gv0)));
     TRY(launch(c, KID_VOLUME, [&] {
         if (c->params.precision == DSEM_PREC_FP32)
-            k_volume<P, 1><<<K, vblock<P, 1>(), 0, s>>>(JQ, m, ph, c->QA, muh, nut, c->FV, c->Qtp, slag);
+            k_volume<P, 1><<<gv1, vblock<P, 1>(), 0, s>>>(JQ, m, ph, c->QA, muh, nut, c->FV, c->Qtp, slag);
         else if (c->params.precision == DSEM_PREC_BF16X3)
-            k_volume<P, 2><<<K, vblock<P, 2>(), 0, s>>>(JQ, m, ph, c->QA, muh, nut, c->FV, c->Qtp, slag);
+            k_volume<P, 2><<<gv2, vblock<P, 2>(), 0, s>>>(JQ, m, ph, c->QA, muh, nut, c->FV, c->Qtp, slag);
         else
-            k_volume<P, 0><<<K, vblock<P, 0>(), 0, s>>>(JQ, m, ph, c->QA, muh, nut, c->FV, c->Qtp, slag);
+            k_volume<P, 0><<<gv0, vblock<P, 0>(), 0, s>>>(JQ, m, ph, c->QA, muh, nut, c->FV, c->Qtp, slag);
     }));
     TRY(launch(c, KID_FACE_VISC, [&] { k_face_visc<P><<<face_blocks, BLOCK, 0, s>>>(c->FV, c->FI, m, ph, c->FT); }));
     if (make_traces)
