@@ -181,6 +181,27 @@ def test_full_size_rhs_matches_oracle(metrics):
     assert np.all(err <= TOL), err
 
 
+@pytest.mark.parametrize("metrics", ["per_point", "per_element"])
+def test_rhs_more_elements_than_resident_workgroups(metrics):
+    """7^3 elements at p=7: the FP64 volume kernel at p >= 7 keeps one
+    workgroup per CU resident and loops over elements, copying the next
+    element's inputs into LDS during the current one.  343 elements on 256
+    CUs: some workgroups take two elements, some one."""
+    print(f"\n[case] rhs distorted 7^3 p=7 metrics={metrics}", flush=True)
+    mesh = distorted_box(7, 7, amp=0.05, seed=77) if metrics == "per_point" else M.periodic_box(7, 7)
+    g, JQ = _case(mesh, perturb=1e-2, seed=7)
+    if metrics == "per_element":
+        per_element_metrics(g)
+    ph = O.phys(re=400.0)
+    want = O.rhs(g, JQ, ph)
+    with DSEM(mesh, re=400.0, metrics=metrics) as s:
+        s.set_state(JQ)
+        got = s.rhs()
+    err = rel_l2(got, want)
+    print("rel L2 per field", err, flush=True)
+    assert np.all(err <= TOL), err
+
+
 @pytest.mark.parametrize("p", [1, 2, 5, 7])
 def test_rhs_distorted_mesh_every_order(p):
     """Non-affine elements at the orders not covered above."""
