@@ -948,8 +948,21 @@ __global__ __launch_bounds__((vblock<P, PM>())) void k_volume(const double *__re
 #pragma unroll
             for (int r = 0; r < R; ++r) {
                 const int pt = tid + r * VBLOCK;
+                if constexpr (PM != 0) {
+                    // mixed modes: unconditional loads from a clamped point,
+                    // converted after the select (guarded, converting loads:
+                    // 1.11 ms at 22^3 p=8 FP32, these: 0.98 ms; the FP64
+                    // kernel is faster with the guarded form)
+                    const int pc = pt < S::NP ? pt : S::NP - 1;
 #pragma unroll
-                for (int i = 0; i < 9; ++i) gpre[r][i] = pt < S::NP ? (CA)gg[i * npm + pt * m.ms] : (CA)0;
+                    for (int i = 0; i < 9; ++i) {
+                        const double g = gg[i * npm + pc * m.ms];
+                        gpre[r][i] = (CA)(pt < S::NP ? g : 0.0);
+                    }
+                } else {
+#pragma unroll
+                    for (int i = 0; i < 9; ++i) gpre[r][i] = pt < S::NP ? (CA)gg[i * npm + pt * m.ms] : (CA)0;
+                }
             }
         }
         if constexpr (PM != 0) {
