@@ -543,6 +543,18 @@ __device__ __forceinline__ LaneFlux volume_grid(const DevMesh &m, const DevPhys 
         const bool face = (jc == 0 || jc == S::NC - 1);
         const int fc = face_of<P>(D, jc);
         const int mo = e2m_e[fc - 1] >> 1;
+        // Vector form: the point's metric row, and its line's face average and
+        // norm (every lane: face_of names a face for any j, and interior lanes
+        // hit the face lane's cache lines), requested before the contractions
+        // instead of one wait each at their use
+        double vg[3] = {0.0, 0.0, 0.0}, vqa[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, vnrm = 1.0;
+        if constexpr (!MF) {
+#pragma unroll
+            for (int i = 0; i < 3; ++i) vg[i] = gmd[i * nsm + s * m.ms];
+#pragma unroll
+            for (int v = 0; v < 5; ++v) vqa[v] = QA[((size_t)mo * NEQ + v) * S::NF + q];
+            vnrm = m.norm[(size_t)mo * nfm + q * m.ms];
+        }
         double Qv[5];
         CT Bj[MF ? 1 : S::NCG];
         if constexpr (MF) {
@@ -557,7 +569,7 @@ __device__ __forceinline__ LaneFlux volume_grid(const DevMesh &m, const DevPhys 
 #pragma unroll
             for (int v = 0; v < 5; ++v) {
                 if (face) {
-                    Qv[v] = QA[((size_t)mo * NEQ + v) * S::NF + q];
+                    Qv[v] = vqa[v];
                 } else {
                     CA u = 0;
 #pragma unroll
@@ -617,10 +629,9 @@ __device__ __forceinline__ LaneFlux volume_grid(const DevMesh &m, const DevPhys 
             double amu = 1.0;
             if (nut) amu = amu + (MF ? mf_nut : nut[((size_t)e * 3 + D) * S::NS + s]) * Qv[0];   // static Smagorinsky
             if (muh) amu = amu + (MF ? mf_muh : muh[((size_t)e * 3 + D) * S::NS + s]);
-            const int sq = s * m.ms;
-            const double g0 = MF ? mf_g[0] : gmd[0 * nsm + sq];
-            const double g1 = MF ? mf_g[1] : gmd[1 * nsm + sq];
-            const double g2 = MF ? mf_g[2] : gmd[2 * nsm + sq];
+            const double g0 = MF ? mf_g[0] : vg[0];
+            const double g1 = MF ? mf_g[1] : vg[1];
+            const double g2 = MF ? mf_g[2] : vg[2];
             double rv[5];
             {
                 // Stokes stress and Fourier heat flux (x/y/z_vis_flux, PhysLib.f90:464-470)
@@ -640,7 +651,7 @@ __device__ __forceinline__ LaneFlux volume_grid(const DevMesh &m, const DevPhys 
             if (face) {
                 // r / norm as one reciprocal and five products (a rounding apart
                 // from five divisions, which every wave with a face lane paid)
-                const double rn = 1.0 / (MF ? mf_nrm : m.norm[(size_t)mo * nfm + q * m.ms]);
+                const double rn = 1.0 / (MF ? mf_nrm : vnrm);
 #pragma unroll
                 for (int v = 0; v < 5; ++v) FV[(((size_t)e * 6 + fc - 1) * NEQ + v) * S::NF + q] = rv[v] * rn;
             } else {
