@@ -1393,7 +1393,22 @@ __global__ __launch_bounds__(BLOCK) void k_ev(const double *__restrict__ JQ, con
     const int e = blockIdx.x, tid = threadIdx.x;
     const double gamma = ph.gamma;
     const double *src = Slag + (size_t)e * 3 * NEQ * S::NS;
-    for (int i = tid; i < 3 * NEQ * S::NS; i += BLOCK) sS[i] = src[i];
+    {
+        // every load in flight before the first store (a load-store loop
+        // waited out one memory latency per trip)
+        constexpr int NSS = 3 * NEQ * S::NS, NI = (NSS + BLOCK - 1) / BLOCK;
+        double t[NI];
+#pragma unroll
+        for (int u = 0; u < NI; ++u) {
+            const int i = tid + u * BLOCK;
+            t[u] = src[i < NSS ? i : NSS - 1];
+        }
+#pragma unroll
+        for (int u = 0; u < NI; ++u) {
+            const int i = tid + u * BLOCK;
+            if (i < NSS) sS[i] = t[u];
+        }
+    }
     __syncthreads();
     for (int i = tid; i < 3 * S::NS; i += BLOCK) {
         const int d = i / S::NS, s2 = i % S::NS;
