@@ -356,15 +356,6 @@ __global__ __launch_bounds__(BLOCK) void k_face_inv(const double *__restrict__ T
     }
 }
 
-template <int P>
-__device__ __forceinline__ void load_physical_state(const double *__restrict__ JQ, const DevMesh &m, int e, double *sQ)
-{
-    using S = Sz<P>;
-    const double *q = JQ + (size_t)e * NEQ * S::NP;
-    const double *jac = m.jacob + (size_t)e * (m.ms ? S::NP : 1);
-    for (int i = threadIdx.x; i < NEQ * S::NP; i += BLOCK) sQ[i] = q[i] * jac[(i % S::NP) * m.ms];
-}
-
 // Line contractions of the FP64 volume kernel on the matrix cores.
 // v_mfma_f64_4x4x4f64 runs four independent 4x4x4 products per wave.  Lane l
 // (c = l%4, blk = (l/4)%4, r = l/16) supplies A_blk[c][r] and B_blk[r][c]
@@ -1369,6 +1360,29 @@ __device__ __forceinline__ double ev_smooth_point(const double *mu, int kind, in
     return TWELVETH * (6.0 * MU(n, m, l) + MU(n + 1, m, l) + MU(n - 1, m, l) + MU(n, m + 1, l) + MU(n, m - 1, l) + MU(n, m, l + 1) + MU(n, m, l - 1));
 }
 
+// The first NV fields of each staggered grid of one element's lagged-state
+// block (src = [3][NEQ][ns]) into LDS as [3][NV][ns], with every load in
+// flight before the first store (a load-store loop waits out one memory
+// latency per trip).
+template <int P, int NV>
+__device__ __forceinline__ void stage_staggered(const double *__restrict__ src, double *sS, int tid)
+{
+    using S = Sz<P>;
+    constexpr int N = 3 * NV * S::NS, NI = (N + BLOCK - 1) / BLOCK;
+    double t[NI];
+#pragma unroll
+    for (int u = 0; u < NI; ++u) {
+        const int i = tid + u * BLOCK, ic = i < N ? i : N - 1;
+        const int d = ic / (NV * S::NS), r = ic % (NV * S::NS);
+        t[u] = src[d * NEQ * S::NS + r];
+    }
+#pragma unroll
+    for (int u = 0; u < NI; ++u) {
+        const int i = tid + u * BLOCK;
+        if (i < N) sS[i] = t[u];
+    }
+}
+
 // Entropy (EV_routines/Main/Driver.f90:21-60) for one element:
 //   RHS_ent_trans (RHS.f90:21-190) from the lagged face-averaged staggered
 //   states Slag and the current state: D = (Phi + Gamma)/(T Re) and the
@@ -1392,23 +1406,7 @@ __global__ __launch_bounds__(BLOCK) void k_ev(const double *__restrict__ JQ, con
     __shared__ double sRed[2 * BLOCK];
     const int e = blockIdx.x, tid = threadIdx.x;
     const double gamma = ph.gamma;
-    const double *src = Slag + (size_t)e * 3 * NEQ * S::NS;
-    {
-        // every load in flight before the first store (a load-store loop
-        // waited out one memory latency per trip)
-        constexpr int NSS = 3 * NEQ * S::NS, NI = (NSS + BLOCK - 1) / BLOCK;
-        double t[NI];
-#pragma unroll
-        for (int u = 0; u < NI; ++u) {
-            const int i = tid + u * BLOCK;
-            t[u] = src[i < NSS ? i : NSS - 1];
-        }
-#pragma unroll
-        for (int u = 0; u < NI; ++u) {
-            const int i = tid + u * BLOCK;
-            if (i < NSS) sS[i] = t[u];
-        }
-    }
+    stage_staggered<P, NEQ>(Slag + (size_t)e * 3 * NEQ * S::NS, sS, tid);
     __syncthreads();
     for (int i = tid; i < 3 * S::NS; i += BLOCK) {
         const int d = i / S::NS, s2 = i % S::NS;
@@ -1573,11 +1571,7 @@ __global__ __launch_bounds__(BLOCK) void k_smag(const double *__restrict__ JQ, c
     __shared__ double sS[3 * 4 * S::NS];
     __shared__ double sNu[S::NP];
     const int e = blockIdx.x, tid = threadIdx.x;
-    const double *src = Slag + (size_t)e * 3 * NEQ * S::NS;
-    for (int i = tid; i < 3 * 4 * S::NS; i += BLOCK) {
-        const int d = i / (4 * S::NS), r = i % (4 * S::NS);
-        sS[i] = src[d * NEQ * S::NS + r];
-    }
+    stage_staggered<P, 4>(Slag + (size_t)e * 3 * NEQ * S::NS, sS, tid);
     __syncthreads();
     const int npm = m.ms ? S::NP : 1;
     const double *gg = m.gmetg + (size_t)e * 9 * npm;
@@ -1651,13 +1645,7 @@ __global__ __launch_bounds__(BLOCK) void k_diag(const double *__restrict__ JQ, c
     __shared__ double sRed[5][BLOCK];
     const int e = blockIdx.x, tid = threadIdx.x;
     const bool vort = Slag != nullptr;
-    if (vort) {
-        const double *src = Slag + (size_t)e * 3 * NEQ * S::NS;
-        for (int i = tid; i < 3 * 4 * S::NS; i += BLOCK) {
-            const int d = i / (4 * S::NS), r = i % (4 * S::NS);
-            sS[i] = src[d * NEQ * S::NS + r];
-        }
-    }
+    if (vort) stage_staggered<P, 4>(Slag + (size_t)e * 3 * NEQ * S::NS, sS, tid);
     __syncthreads();
     const int npm = m.ms ? S::NP : 1;
     const double *gg = m.gmetg + (size_t)e * 9 * npm;
