@@ -1364,6 +1364,19 @@ __device__ __forceinline__ double ev_smooth_point(const double *mu, int kind, in
 // block (src = [3][NEQ][ns]) into LDS as [3][NV][ns], with every load in
 // flight before the first store (a load-store loop waits out one memory
 // latency per trip).
+// D (ncg x nc) and B (nc x ncg) of this order into LDS: the contractions of
+// the per-point passes read them with per-work-item rows, which from the
+// constant tables are vector memory loads with their own latency.
+template <int P>
+__device__ __forceinline__ void stage_ops(double *sDm, double *sBm, int tid)
+{
+    using S = Sz<P>;
+    for (int i = tid; i < S::NCG * S::NC; i += BLOCK) {
+        sDm[i] = c_D[P][i];
+        sBm[i] = c_B[P][i];
+    }
+}
+
 template <int P, int NV>
 __device__ __forceinline__ void stage_staggered(const double *__restrict__ src, double *sS, int tid)
 {
@@ -1404,9 +1417,11 @@ __global__ __launch_bounds__(BLOCK) void k_ev(const double *__restrict__ JQ, con
     __shared__ double sT[3 * S::NS];
     __shared__ double sMu[S::NP];
     __shared__ double sRed[2 * BLOCK];
+    __shared__ double sDm[S::NCG * S::NC], sBm[S::NC * S::NCG];
     const int e = blockIdx.x, tid = threadIdx.x;
     const double gamma = ph.gamma;
     stage_staggered<P, NEQ>(Slag + (size_t)e * 3 * NEQ * S::NS, sS, tid);
+    stage_ops<P>(sDm, sBm, tid);
     __syncthreads();
     for (int i = tid; i < 3 * S::NS; i += BLOCK) {
         const int d = i / S::NS, s2 = i % S::NS;
@@ -1438,7 +1453,7 @@ __global__ __launch_bounds__(BLOCK) void k_ev(const double *__restrict__ JQ, con
                 const double *fld = (f < 5) ? sS + (d * NEQ + f) * S::NS : sT + d * S::NS;
                 double t = 0.0;
 #pragma unroll
-                for (int j = 0; j < S::NC; ++j) t = fma(c_D[P][k * S::NC + j], fld[sidx<P>(d, j, a, b)], t);
+                for (int j = 0; j < S::NC; ++j) t = fma(sDm[k * S::NC + j], fld[sidx<P>(d, j, a, b)], t);
                 w[d] = t;
             }
 #pragma unroll
@@ -1550,7 +1565,7 @@ __global__ __launch_bounds__(BLOCK) void k_ev(const double *__restrict__ JQ, con
         else { a = s2 % S::NCG; b = (s2 / S::NCG) % S::NCG; j = s2 / (S::NCG * S::NCG); }
         double t = 0.0;
 #pragma unroll
-        for (int k = 0; k < S::NCG; ++k) t = fma(c_B[P][j * S::NCG + k], sMu[gidx<P>(d, k, a, b)], t);
+        for (int k = 0; k < S::NCG; ++k) t = fma(sBm[j * S::NCG + k], sMu[gidx<P>(d, k, a, b)], t);
         muh[(size_t)e * 3 * S::NS + i] = t;
     }
 }
@@ -1570,8 +1585,10 @@ __global__ __launch_bounds__(BLOCK) void k_smag(const double *__restrict__ JQ, c
     using S = Sz<P>;
     __shared__ double sS[3 * 4 * S::NS];
     __shared__ double sNu[S::NP];
+    __shared__ double sDm[S::NCG * S::NC], sBm[S::NC * S::NCG];
     const int e = blockIdx.x, tid = threadIdx.x;
     stage_staggered<P, 4>(Slag + (size_t)e * 3 * NEQ * S::NS, sS, tid);
+    stage_ops<P>(sDm, sBm, tid);
     __syncthreads();
     const int npm = m.ms ? S::NP : 1;
     const double *gg = m.gmetg + (size_t)e * 9 * npm;
@@ -1593,7 +1610,7 @@ __global__ __launch_bounds__(BLOCK) void k_smag(const double *__restrict__ JQ, c
                 const double *fld = sS + (d * 4 + f) * S::NS;
                 double t = 0.0;
 #pragma unroll
-                for (int j = 0; j < S::NC; ++j) t = fma(c_D[P][k * S::NC + j], fld[sidx<P>(d, j, a, b)], t);
+                for (int j = 0; j < S::NC; ++j) t = fma(sDm[k * S::NC + j], fld[sidx<P>(d, j, a, b)], t);
                 wd[d] = t;
             }
 #pragma unroll
@@ -1623,7 +1640,7 @@ __global__ __launch_bounds__(BLOCK) void k_smag(const double *__restrict__ JQ, c
         else { a = s2 % S::NCG; b = (s2 / S::NCG) % S::NCG; j = s2 / (S::NCG * S::NCG); }
         double t = 0.0;
 #pragma unroll
-        for (int k = 0; k < S::NCG; ++k) t = fma(c_B[P][j * S::NCG + k], sNu[gidx<P>(d, k, a, b)], t);
+        for (int k = 0; k < S::NCG; ++k) t = fma(sBm[j * S::NCG + k], sNu[gidx<P>(d, k, a, b)], t);
         nut[(size_t)e * 3 * S::NS + i] = t < 0.0 ? 0.0 : t;
     }
 }
@@ -1643,9 +1660,13 @@ __global__ __launch_bounds__(BLOCK) void k_diag(const double *__restrict__ JQ, c
     using S = Sz<P>;
     __shared__ double sS[3 * 4 * S::NS];
     __shared__ double sRed[5][BLOCK];
+    __shared__ double sDm[S::NCG * S::NC], sBm[S::NC * S::NCG];
     const int e = blockIdx.x, tid = threadIdx.x;
     const bool vort = Slag != nullptr;
-    if (vort) stage_staggered<P, 4>(Slag + (size_t)e * 3 * NEQ * S::NS, sS, tid);
+    if (vort) {
+        stage_staggered<P, 4>(Slag + (size_t)e * 3 * NEQ * S::NS, sS, tid);
+        stage_ops<P>(sDm, sBm, tid);
+    }
     __syncthreads();
     const int npm = m.ms ? S::NP : 1;
     const double *gg = m.gmetg + (size_t)e * 9 * npm;
@@ -1674,7 +1695,7 @@ __global__ __launch_bounds__(BLOCK) void k_diag(const double *__restrict__ JQ, c
                     const double *fld = sS + (d * 4 + f) * S::NS;
                     double t = 0.0;
 #pragma unroll
-                    for (int j = 0; j < S::NC; ++j) t = fma(c_D[P][k * S::NC + j], fld[sidx<P>(d, j, a, b)], t);
+                    for (int j = 0; j < S::NC; ++j) t = fma(sDm[k * S::NC + j], fld[sidx<P>(d, j, a, b)], t);
                     wd[d] = t;
                 }
 #pragma unroll
