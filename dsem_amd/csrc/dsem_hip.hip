@@ -1416,6 +1416,7 @@ __global__ __launch_bounds__(BLOCK) void k_ev(const double *__restrict__ JQ, con
     __shared__ double sS[3 * NEQ * S::NS];
     __shared__ double sT[3 * S::NS];
     __shared__ double sMu[S::NP];
+    __shared__ double sH[S::NP];   // h of each point, reused by the cap
     __shared__ double sRed[2 * BLOCK];
     __shared__ double sDm[S::NCG * S::NC], sBm[S::NC * S::NCG];
     const int e = blockIdx.x, tid = threadIdx.x;
@@ -1482,6 +1483,7 @@ __global__ __launch_bounds__(BLOCK) void k_ev(const double *__restrict__ JQ, con
         if (resid_out) resid_out[(size_t)e * S::NP + pt] = rhs;
         if (ducros_out) ducros_out[(size_t)e * S::NP + pt] = duc;
         const double h = pow(fabs(1.0 / jc), 1.0 / 3.0) / c3;
+        sH[pt] = h;
         double mur = ev.cmu * ph.re * rho * (h * h) * fabs(rhs);
         if (ev.ducros) mur = mur * duc;
         sMu[pt] = mur;
@@ -1506,8 +1508,7 @@ __global__ __launch_bounds__(BLOCK) void k_ev(const double *__restrict__ JQ, con
     __syncthreads();
     for (int pt = tid; pt < S::NP; pt += BLOCK) {
         if (ev.cmax_cap) {
-            const double h = pow(fabs(1.0 / jac[pt * m.ms]), 1.0 / 3.0) / c3;
-            cap = ev.cmax * sRed[0] * h * sRed[BLOCK];
+            cap = ev.cmax * sRed[0] * sH[pt] * sRed[BLOCK];
         }
         sMu[pt] = fmin(fmax(sMu[pt], 0.0), cap);
     }
