@@ -20,8 +20,9 @@ DSEM_ERR_UNSUPPORTED = -4
 DSEM_ERR_NOMEM = -5
 
 PREC_FP64, PREC_FP32, PREC_BF16X3 = 0, 1, 2
-METRICS_AUTO, METRICS_PER_POINT, METRICS_PER_ELEMENT = 0, 1, 2
+METRICS_PER_POINT, METRICS_AUTO, METRICS_PER_ELEMENT = 0, 1, 2
 EV_DUCROS, EV_SMOOTHING, EV_CMAX_CAP = 1, 2, 4
+LES_SHOCK_SENSOR, LES_RHO_SENSOR = 1, 2
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int32)
@@ -90,6 +91,9 @@ def load() -> C.CDLL:
         "dsem_keep_lagged_states": ([vp, C.c_int], C.c_int),
         "dsem_set_les": ([vp, C.c_int, C.c_double, C.c_double], C.c_int),
         "dsem_les_update": ([vp, _dp, _dp], C.c_int),
+        "dsem_set_les_sensors": ([vp, C.c_int], C.c_int),
+        "dsem_get_ev_sensors": ([vp, _dp, _dp], C.c_int),
+        "dsem_modal_filter": ([vp, C.c_int], C.c_int),
         "dsem_set_les_viscosity": ([vp, _dp], C.c_int),
         "dsem_rhs_vjp": ([vp, _dp, _dp, _dp], C.c_int),
         "dsem_rhs_vjp_device": ([vp, vp, vp, vp, _dp], C.c_int),

@@ -69,6 +69,11 @@ __constant__ float c_Df[9][90];
 __constant__ float c_Bf[9][90];
 __constant__ float c_Mf[9][81];
 __constant__ double c_Dint[9][9];  // sum over the interior Lobatto j of D[k][j]
+// modal filter: unnormalised DCT-II (FFTW REDFT10) and DCT-III (REDFT01) of
+// length N = ncg, C2[k*N + j] = 2 cos(pi (j + 1/2) k / N),
+// C3[k*N + j] = (j == 0 ? 1 : 2 cos(pi j (k + 1/2) / N))
+__constant__ double c_dct2[9][81];
+__constant__ double c_dct3[9][81];
 
 // Contraction arithmetic of the volume kernel, by precision mode:
 //   PM 0  FP64 (the drop-in path);
@@ -736,9 +741,9 @@ typedef __attribute__((address_space(3))) void *LdsPtr;
 __device__ __forceinline__ void lds_dword(const void *src, void *lds_wave_base)
 {
     const unsigned l = __builtin_amdgcn_readfirstlane((unsigned)(size_t)(LdsPtr)lds_wave_base);
-    // M0 is reserved by the compiler, which sets it right before each
-    // instruction of its own that reads it (the volume kernel has none)
-    asm volatile("s_mov_b32 m0, %1\n\ts_nop 0\n\tglobal_load_lds_dword %0, off" : : "v"(src), "s"(l) : "memory");
+    // M0 is declared clobbered, so the compiler keeps no live value of its
+    // own in it across this copy
+    asm volatile("s_mov_b32 m0, %1\n\ts_nop 0\n\tglobal_load_lds_dword %0, off" : : "v"(src), "s"(l) : "memory", "m0");
 }
 
 // s_waitcnt immediate of gfx9 for vmcnt(0) alone (expcnt 7, lgkmcnt 15).
@@ -1410,7 +1415,7 @@ template <int P>
 __global__ __launch_bounds__(BLOCK) void k_ev(const double *__restrict__ JQ, const double *__restrict__ Slag, DevMesh m,
                                                DevPhys ph, EvArgs ev, double *__restrict__ muh,
                                                double *__restrict__ muhg_out, double *__restrict__ resid_out,
-                                               double *__restrict__ ducros_out)
+                                               double *__restrict__ ducros_out, double *__restrict__ rhos_out)
 {
     using S = Sz<P>;
     __shared__ double sS[3 * NEQ * S::NS];
@@ -1482,6 +1487,15 @@ __global__ __launch_bounds__(BLOCK) void k_ev(const double *__restrict__ JQ, con
         const double duc = fmax(0.0, fmin(1.0, (s1 * s1 / (s1 * s1 + cur * cur + 1.0e-9))));
         if (resid_out) resid_out[(size_t)e * S::NP + pt] = rhs;
         if (ducros_out) ducros_out[(size_t)e * S::NP + pt] = duc;
+        if (rhos_out) {
+            // density-based sensor of calculate_terms (RHS.f90:263-275)
+            const double eps = 1.0e-9;
+            const double rx2 = gr[0][0] * gr[0][0], ry2 = gr[0][1] * gr[0][1], rz2 = gr[0][2] * gr[0][2];
+            rhos_out[(size_t)e * S::NP + pt] =
+                sqrt((1.0 / ((1.0 / (rx2 + eps)) + (1.0 / (ry2 + eps)) + 1.0)) +
+                     (1.0 / ((1.0 / (rx2 + eps)) + (1.0 / (rz2 + eps)) + 1.0)) +
+                     (1.0 / ((1.0 / (ry2 + eps)) + (1.0 / (rz2 + eps)) + 1.0)));
+        }
         const double h = pow(fabs(1.0 / jc), 1.0 / 3.0) / c3;
         sH[pt] = h;
         double mur = ev.cmu * ph.re * rho * (h * h) * fabs(rhs);
@@ -1515,17 +1529,34 @@ __global__ __launch_bounds__(BLOCK) void k_ev(const double *__restrict__ JQ, con
     __syncthreads();
     if (ev.smoothing) {
         constexpr int N = S::NCG;
-        // corners (independent of each other): all eight read, then write
-        double cval = 0.0;
-        int cidx = -1;
-        if (tid < 8) {
-            const int n = (tid & 1) ? N : 1, mm = (tid & 2) ? N : 1, l = (tid & 4) ? N : 1;
-            cval = ev_smooth_point<P>(sMu, 0, n, mm, l);
-            cidx = (n - 1) + N * ((mm - 1) + N * (l - 1));
+        if constexpr (N == 2) {
+            // p = 1: every point is a corner and each corner neighbours three
+            // others, so one work-item updates them in the reference's order
+            // (viscosity_smoothing.f90:31-93)
+            if (tid == 0) {
+                constexpr int cn[8][3] = {{1, 1, 1}, {N, 1, 1}, {1, N, 1}, {1, 1, N},
+                                          {1, N, N}, {N, 1, N}, {N, N, 1}, {N, N, N}};
+#pragma unroll 1
+                for (int c = 0; c < 8; ++c) {
+                    const int n = cn[c][0], mm = cn[c][1], l = cn[c][2];
+                    sMu[(n - 1) + N * ((mm - 1) + N * (l - 1))] = ev_smooth_point<P>(sMu, 0, n, mm, l);
+                }
+            }
+            __syncthreads();
+        } else {
+            // corners (no two neighbour each other when N > 2): all eight
+            // read, then write
+            double cval = 0.0;
+            int cidx = -1;
+            if (tid < 8) {
+                const int n = (tid & 1) ? N : 1, mm = (tid & 2) ? N : 1, l = (tid & 4) ? N : 1;
+                cval = ev_smooth_point<P>(sMu, 0, n, mm, l);
+                cidx = (n - 1) + N * ((mm - 1) + N * (l - 1));
+            }
+            __syncthreads();
+            if (cidx >= 0) sMu[cidx] = cval;
+            __syncthreads();
         }
-        __syncthreads();
-        if (cidx >= 0) sMu[cidx] = cval;
-        __syncthreads();
         // faces: the six faces are independent; within a face, lexicographic
         // Gauss-Seidel equals a sweep over the diagonals u+v = h
         if (N > 2) {
@@ -1576,11 +1607,13 @@ __global__ __launch_bounds__(BLOCK) void k_ev(const double *__restrict__ JQ, con
 // element: Gauss-point gradients of the lagged staggered states,
 // S = (2 S_ij S_ij)^(1/2), nu_t = Re (C_s Delta)^2 S clamped to [0, numax],
 // prolonged to the three staggered grids with negative values set to zero.
-// As in filteredStrain, rho = (JQ)_1 jacob of the element's first point and
+// The shock and density sensors (Gauss-point fields left by k_ev) scale
+// nu_t before the clamp when their switches are set.  As in filteredStrain, rho = (JQ)_1 jacob of the element's first point and
 // u = (JQ)_2 / rho.
 template <int P>
 __global__ __launch_bounds__(BLOCK) void k_smag(const double *__restrict__ JQ, const double *__restrict__ Slag,
-                                                 DevMesh m, DevPhys ph, double cs, double numax,
+                                                 DevMesh m, DevPhys ph, double cs, double numax, int sensors,
+                                                 const double *__restrict__ ducros, const double *__restrict__ rhos,
                                                  double *__restrict__ nut, double *__restrict__ nut_g)
 {
     using S = Sz<P>;
@@ -1628,7 +1661,10 @@ __global__ __launch_bounds__(BLOCK) void k_smag(const double *__restrict__ JQ, c
         const double Sbar = sqrt(2.0 * ux * ux + 2.0 * S12 * S12 + 2.0 * S13 * S13 + 2.0 * S12 * S12 + 2.0 * vy * vy +
                                  2.0 * S23 * S23 + 2.0 * S13 * S13 + 2.0 * S23 * S23 + 2.0 * wz * wz);
         const double delta = pow(fabs(1.0 / jac[pq]), 1.0 / 3.0) / c3;
-        const double nt = ph.re * (cs * delta) * (cs * delta) * Sbar;
+        double nt = ph.re * (cs * delta) * (cs * delta) * Sbar;
+        // shockSensor / rhoSensor switches (staticSmag.f90:68-69), before the clamp
+        if (sensors & DSEM_LES_SHOCK_SENSOR) nt = nt * (1.0 - ducros[(size_t)e * S::NP + pt]);
+        if (sensors & DSEM_LES_RHO_SENSOR) nt = nt * rhos[(size_t)e * S::NP + pt];
         sNu[pt] = fmin(fmax(nt, 0.0), numax);
         if (nut_g) nut_g[(size_t)e * S::NP + pt] = sNu[pt];
     }
@@ -1780,6 +1816,80 @@ __global__ __launch_bounds__(BLOCK) void k_dtcalc(const double *__restrict__ JQ,
     atomicMin(out, (unsigned long long)__double_as_longlong(best));
 }
 
+// One 1-D transform of the modal filter along direction d, for the three
+// momentum fields of the element in LDS: out = C (in) along d.
+template <int P>
+__device__ __forceinline__ void modal_pass(const double *in, double *out, const double *C, int d, int tid)
+{
+    using S = Sz<P>;
+    for (int i = tid; i < 3 * S::NP; i += BLOCK) {
+        const int v = i / S::NP, pt = i % S::NP;
+        const int n = pt % S::NCG, mi = (pt / S::NCG) % S::NCG, l = pt / (S::NCG * S::NCG);
+        const int k = d == 0 ? n : (d == 1 ? mi : l);
+        const int a = d == 0 ? mi : n;
+        const int b = d == 2 ? mi : l;
+        double t = 0.0;
+#pragma unroll
+        for (int j = 0; j < S::NCG; ++j) t = fma(C[k * S::NCG + j], in[v * S::NP + gidx<P>(d, j, a, b)], t);
+        out[i] = t;
+    }
+}
+
+// modalfilter (src/Energydissipation/energydissipation.f90:720-959), called
+// every 50 steps before tstep (src/3DDriver_parfmdf.f90:462-463), for one
+// element: the momentum fields Q = (J Q) jacob(1,1,1) go to modal space by a
+// 3-D DCT-II, are divided by 2N (mode 0) or N (modes >= 1) per direction,
+// modes with any index >= cutoff are zeroed (the reference keeps them all:
+// its mode edit at :778-783 is commented out, cutoff = N), then divided by 2
+// per direction for modes >= 1 and brought back by a 3-D DCT-III, and
+// divided by jacob(1,1,1).  With cutoff = N this is the identity up to
+// round-off.
+template <int P>
+__global__ __launch_bounds__(BLOCK) void k_modal(double *__restrict__ JQ, DevMesh m, int cutoff)
+{
+    using S = Sz<P>;
+    constexpr int N = S::NCG, NQ = 3 * S::NP;
+    __shared__ double sA[NQ], sB[NQ];
+    __shared__ double sC2[N * N], sC3[N * N];
+    const int e = blockIdx.x, tid = threadIdx.x;
+    const double j0 = m.jacob[(size_t)e * (m.ms ? S::NP : 1)];
+    double *q = JQ + ((size_t)e * NEQ + 1) * S::NP;   // rho u, rho v, rho w
+    for (int i = tid; i < NQ; i += BLOCK) sA[i] = q[i] * j0;
+    for (int i = tid; i < N * N; i += BLOCK) {
+        sC2[i] = c_dct2[P][i];
+        sC3[i] = c_dct3[P][i];
+    }
+    __syncthreads();
+    modal_pass<P>(sA, sB, sC2, 0, tid);
+    __syncthreads();
+    modal_pass<P>(sB, sA, sC2, 1, tid);
+    __syncthreads();
+    modal_pass<P>(sA, sB, sC2, 2, tid);
+    __syncthreads();
+    const double n2 = 2.0 * N, n1 = (double)N;
+    for (int i = tid; i < NQ; i += BLOCK) {
+        const int pt = i % S::NP;
+        const int a = pt % N, b = (pt / N) % N, c = pt / (N * N);
+        double x = sB[i];
+        x = x / (a == 0 ? n2 : n1);
+        x = x / (b == 0 ? n2 : n1);
+        x = x / (c == 0 ? n2 : n1);
+        if (a >= cutoff || b >= cutoff || c >= cutoff) x = 0.0;
+        if (a > 0) x = x / 2.0;
+        if (b > 0) x = x / 2.0;
+        if (c > 0) x = x / 2.0;
+        sB[i] = x;
+    }
+    __syncthreads();
+    modal_pass<P>(sB, sA, sC3, 0, tid);
+    __syncthreads();
+    modal_pass<P>(sA, sB, sC3, 1, tid);
+    __syncthreads();
+    modal_pass<P>(sB, sA, sC3, 2, tid);
+    __syncthreads();
+    for (int i = tid; i < NQ; i += BLOCK) q[i] = sA[i] / j0;
+}
+
 #include "adjoint.inc"
 
 }  // namespace
@@ -1792,9 +1902,10 @@ namespace {
 const double kArk[5] = {0.0, -0.4178904745, -1.192151694643, -1.697784692471, -1.514183444257};
 const double kCrk[5] = {0.1496590219993, 0.3792103129999, 0.8229550293869, 0.6994504559488, 0.1530572479681};
 
-enum KernelId { KID_TRACES = 0, KID_FACE_INV, KID_VOLUME, KID_FACE_VISC, KID_UPDATE, KID_DTCALC, KID_EV, KID_DIAG, KID_ADJ, KID_LES, KID_COUNT };
+enum KernelId { KID_TRACES = 0, KID_FACE_INV, KID_VOLUME, KID_FACE_VISC, KID_UPDATE, KID_DTCALC, KID_EV, KID_DIAG, KID_ADJ, KID_LES, KID_MODAL, KID_COUNT };
 const char *const kKernelNames[KID_COUNT] = {"k_traces", "k_face_inv", "k_volume", "k_face_visc",
-                                             "k_update", "k_dtcalc", "k_ev", "k_diag", "k_adj", "k_smag"};
+                                             "k_update", "k_dtcalc", "k_ev", "k_diag", "k_adj", "k_smag",
+                                             "k_modal"};
 
 template <typename T>
 struct DevBuf {
@@ -1826,6 +1937,8 @@ struct dsem_ctx {
     bool les = false;                          // static Smagorinsky model
     double les_cs = 0.0, les_numax = 0.0;
     double *nut = nullptr, *nut_g = nullptr;   // eddy viscosity on the staggered grids / Gauss points
+    int les_sensors = 0;                       // DSEM_LES_SHOCK_SENSOR | DSEM_LES_RHO_SENSOR
+    double *ev_rhos = nullptr;                 // density sensor at the Gauss points (k_ev)
     double *bmat = nullptr, *diag_part = nullptr, *diag_out = nullptr;   // diagnostics
     // adjoint buffers (allocated by the first VJP call)
     double *aQVb = nullptr, *aGVb = nullptr, *aFTbe = nullptr, *aFVb = nullptr, *aQAbe = nullptr;
@@ -2034,9 +2147,12 @@ int run_ev(dsem_ctx *c, bool debug)
     ev.ducros = (c->params.ev_flags & DSEM_EV_DUCROS) != 0;
     ev.smoothing = (c->params.ev_flags & DSEM_EV_SMOOTHING) != 0;
     ev.cmax_cap = (c->params.ev_flags & DSEM_EV_CMAX_CAP) != 0;
+    // the Smagorinsky sensors read the Ducros and density sensors of this call
+    const bool sens = c->les && c->les_sensors;
     return launch(c, KID_EV, [&] {
         k_ev<P><<<c->K, BLOCK, 0, c->stream>>>(c->JQ, c->Slag, m, ph, ev, c->muh, c->muhg, debug ? c->ev_resid : nullptr,
-                                               debug ? c->ev_ducros : nullptr);
+                                               (debug || sens) ? c->ev_ducros : nullptr,
+                                               (debug || sens) ? c->ev_rhos : nullptr);
     });
 }
 
@@ -2064,8 +2180,8 @@ int run_les(dsem_ctx *c, bool gauss)
     const DevMesh m = dev_mesh(c);
     const DevPhys ph = dev_phys(c);
     return launch(c, KID_LES, [&] {
-        k_smag<P><<<c->K, BLOCK, 0, c->stream>>>(c->JQ, c->Slag, m, ph, c->les_cs, c->les_numax, c->nut,
-                                                 gauss ? c->nut_g : nullptr);
+        k_smag<P><<<c->K, BLOCK, 0, c->stream>>>(c->JQ, c->Slag, m, ph, c->les_cs, c->les_numax, c->les_sensors,
+                                                 c->ev_ducros, c->ev_rhos, c->nut, gauss ? c->nut_g : nullptr);
     });
 }
 
@@ -2107,6 +2223,18 @@ int run_diag(dsem_ctx *c, bool vorticity)
 int diag_dispatch(dsem_ctx *c, bool vorticity)
 {
     DISPATCH_P(c, run_diag, vorticity);
+}
+
+template <int P>
+int run_modal(dsem_ctx *c, int cutoff)
+{
+    const DevMesh m = dev_mesh(c);
+    return launch(c, KID_MODAL, [&] { k_modal<P><<<c->K, BLOCK, 0, c->stream>>>(c->JQ, m, cutoff); });
+}
+
+int modal_dispatch(dsem_ctx *c, int cutoff)
+{
+    DISPATCH_P(c, run_modal, cutoff);
 }
 
 int adjoint_alloc(dsem_ctx *c)
@@ -2291,6 +2419,18 @@ int upload_operators(dsem_ctx *c)
     HIP_TRY(c, hipMemcpyToSymbol(HIP_SYMBOL(c_Bf), Bf, sizeof(Bf)));
     HIP_TRY(c, hipMemcpyToSymbol(HIP_SYMBOL(c_Mf), Mf, sizeof(Mf)));
     HIP_TRY(c, hipMemcpyToSymbol(HIP_SYMBOL(c_Dint), Dint, sizeof(Dint)));
+    double C2[9][81] = {}, C3[9][81] = {};
+    const double pi = 3.141592653589793238462643;
+    for (int p = 1; p <= 8; ++p) {
+        const int N = p + 1;
+        for (int k = 0; k < N; ++k)
+            for (int j = 0; j < N; ++j) {
+                C2[p][k * N + j] = 2.0 * cos(pi * (j + 0.5) * k / N);
+                C3[p][k * N + j] = j == 0 ? 1.0 : 2.0 * cos(pi * j * (k + 0.5) / N);
+            }
+    }
+    HIP_TRY(c, hipMemcpyToSymbol(HIP_SYMBOL(c_dct2), C2, sizeof(C2)));
+    HIP_TRY(c, hipMemcpyToSymbol(HIP_SYMBOL(c_dct3), C3, sizeof(C3)));
     return DSEM_OK;
 }
 
@@ -2391,6 +2531,7 @@ int create_impl(const dsem_mesh_desc *mesh, const dsem_params *prm, int device, 
         TRY(dev_alloc(c, &c->Slag, (size_t)c->K * 3 * NEQ * c->ns));
         TRY(dev_alloc(c, &c->ev_resid, (size_t)c->K * c->np));
         TRY(dev_alloc(c, &c->ev_ducros, (size_t)c->K * c->np));
+        TRY(dev_alloc(c, &c->ev_rhos, (size_t)c->K * c->np));
     }
     TRY(dev_alloc(c, &c->d_scalars, 4));
     HIP_TRY(c, hipMemset(c->JQ, 0, sizeof(double) * nst));
@@ -2594,6 +2735,35 @@ int dsem_set_les(dsem_ctx *c, int enable, double cs, double numax)
     return DSEM_OK;
 }
 
+int dsem_set_les_sensors(dsem_ctx *c, int sensors)
+{
+    if (!c || !c->stream || (sensors & ~(DSEM_LES_SHOCK_SENSOR | DSEM_LES_RHO_SENSOR)))
+        return set_err(c, DSEM_ERR_ARG, "set_les_sensors arguments");
+    HIP_TRY(c, hipSetDevice(c->device));
+    if (sensors && !c->ev_ducros) {
+        // without entropy viscosity nothing computes the sensors: they stay
+        // zero, as the reference's never-written d%ducros and d%rhoSensor do
+        TRY(dev_alloc(c, &c->ev_ducros, (size_t)c->K * c->np));
+        TRY(dev_alloc(c, &c->ev_rhos, (size_t)c->K * c->np));
+        HIP_TRY(c, hipMemsetAsync(c->ev_ducros, 0, sizeof(double) * (size_t)c->K * c->np, c->stream));
+        HIP_TRY(c, hipMemsetAsync(c->ev_rhos, 0, sizeof(double) * (size_t)c->K * c->np, c->stream));
+        HIP_TRY(c, hipStreamSynchronize(c->stream));
+    }
+    c->les_sensors = sensors;
+    return DSEM_OK;
+}
+
+int dsem_get_ev_sensors(dsem_ctx *c, double *ducros, double *rho_sensor)
+{
+    if (!c || !c->stream || !c->ev_ducros) return set_err(c, DSEM_ERR_ARG, "get_ev_sensors: no sensors computed");
+    HIP_TRY(c, hipSetDevice(c->device));
+    const size_t n = (size_t)c->K * c->np;
+    if (ducros) HIP_TRY(c, hipMemcpyAsync(ducros, c->ev_ducros, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+    if (rho_sensor) HIP_TRY(c, hipMemcpyAsync(rho_sensor, c->ev_rhos, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return DSEM_OK;
+}
+
 int dsem_les_update(dsem_ctx *c, double *nut_g, double *nut_s)
 {
     if (!c || !c->stream || !c->les) return set_err(c, DSEM_ERR_ARG, "les_update needs dsem_set_les(ctx, 1, ...)");
@@ -2791,6 +2961,17 @@ int dsem_rk_step_vjp(dsem_ctx *c, double dt, double *Qbar, double *nubar)
     HIP_TRY(c, hipMemcpyAsync(c->aOut, Qbar, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
     TRY(dsem_rk_step_vjp_device(c, dt, c->aOut, nubar));
     HIP_TRY(c, hipMemcpyAsync(Qbar, c->aOut, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return DSEM_OK;
+}
+
+int dsem_modal_filter(dsem_ctx *c, int cutoff)
+{
+    if (!c || !c->stream) return set_err(c, DSEM_ERR_ARG, "modal_filter arguments");
+    if (cutoff < 0 || cutoff > c->ncg) return set_err(c, DSEM_ERR_ARG, "modal_filter: cutoff must be in 0..p+1");
+    HIP_TRY(c, hipSetDevice(c->device));
+    c->traces_cur = false;   // the state changes: the next step recomputes its face traces
+    TRY(modal_dispatch(c, cutoff));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     return DSEM_OK;
 }

@@ -19,7 +19,7 @@ module dsem_hip
    integer(c_int), parameter :: DSEM_ERR_UNSUPPORTED = -4
    integer(c_int), parameter :: DSEM_ERR_NOMEM = -5
    integer(c_int), parameter :: DSEM_PREC_FP64 = 0, DSEM_PREC_FP32 = 1, DSEM_PREC_BF16X3 = 2
-   integer(c_int), parameter :: DSEM_METRICS_AUTO = 0, DSEM_METRICS_PER_POINT = 1, DSEM_METRICS_PER_ELEMENT = 2
+   integer(c_int), parameter :: DSEM_METRICS_PER_POINT = 0, DSEM_METRICS_AUTO = 1, DSEM_METRICS_PER_ELEMENT = 2
    integer(c_int), parameter :: DSEM_EV_DUCROS = 1, DSEM_EV_SMOOTHING = 2, DSEM_EV_CMAX_CAP = 4
 
    type, bind(C) :: dsem_mesh_desc
@@ -119,6 +119,22 @@ module dsem_hip
          type(c_ptr), value :: ctx
          integer(c_int), value :: enable
       end function dsem_keep_lagged_states
+
+      ! Smagorinsky sensor switches (input keywords shockSensor, rhoSensor):
+      ! DSEM_LES_SHOCK_SENSOR = 1, DSEM_LES_RHO_SENSOR = 2
+      integer(c_int) function dsem_set_les_sensors(ctx, sensors) bind(C, name='dsem_set_les_sensors')
+         import :: c_int, c_ptr
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: sensors
+      end function dsem_set_les_sensors
+
+      ! modalfilter(d,ngridl,time,k), called by the driver when mod(k,50) == 0
+      ! (src/3DDriver_parfmdf.f90:462-463); cutoff = p+1 keeps every mode
+      integer(c_int) function dsem_modal_filter(ctx, cutoff) bind(C, name='dsem_modal_filter')
+         import :: c_int, c_ptr
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: cutoff
+      end function dsem_modal_filter
    end interface
 
 contains

@@ -38,7 +38,7 @@ def _ip(a: np.ndarray):
 class DSEM:
     def __init__(self, mesh: Mesh, gamma: float = 1.4, re: float = 1600.0, pr: float = 0.71,
                  mach: float = 1.0, shock: bool = False, precision: str = "fp64",
-                 metrics: str = "auto", device: int = 0, cmu: float = 1.0, mumax: float = 1.0,
+                 metrics: str = "per_point", device: int = 0, cmu: float = 1.0, mumax: float = 1.0,
                  cmax: float = 0.5, ducros: bool = False, smoothing: bool = False, cmax_cap: bool = False):
         self._L = L.load()
         self.mesh = mesh
@@ -157,6 +157,24 @@ class DSEM:
     def set_les(self, enable: bool = True, cs: float = 0.17, numax: float = 1.0) -> None:
         """Static Smagorinsky model (staticSmag.f90): nu_t = Re (C_s Delta)^2 |S|."""
         self._check(self._L.dsem_set_les(self._h, int(enable), cs, numax))
+
+    def set_les_sensors(self, shock_sensor: bool = False, rho_sensor: bool = False) -> None:
+        """Smagorinsky sensor switches (shockSensor / rhoSensor, staticSmag.f90:68-69)."""
+        flags = (L.LES_SHOCK_SENSOR if shock_sensor else 0) | (L.LES_RHO_SENSOR if rho_sensor else 0)
+        self._check(self._L.dsem_set_les_sensors(self._h, flags))
+
+    def ev_sensors(self):
+        """(Ducros sensor, density sensor) at the Gauss points [K,np] from the
+        last entropy-viscosity update."""
+        d = np.zeros((self.K, self.np))
+        r = np.zeros((self.K, self.np))
+        self._check(self._L.dsem_get_ev_sensors(self._h, _dp(d), _dp(r)))
+        return d, r
+
+    def modal_filter(self, cutoff: int | None = None) -> None:
+        """modalfilter (energydissipation.f90:720-959) on the state; cutoff None
+        keeps every mode, as the reference does (the identity up to round-off)."""
+        self._check(self._L.dsem_modal_filter(self._h, self.ncg if cutoff is None else int(cutoff)))
 
     def les_update(self):
         """(nu_t at the Gauss points [K, np], nu_t on the staggered grids [K, 3, ns])

@@ -58,9 +58,15 @@ extern "C" {
 #define DSEM_PREC_FP32   1   /* FP32 contractions, FP64 state/RK/reductions  */
 #define DSEM_PREC_BF16X3 2   /* BF16 3-term split contractions, FP32 accum   */
 
-#define DSEM_METRICS_AUTO        0  /* per element when every element is affine */
-#define DSEM_METRICS_PER_POINT   1  /* reference: metrics at every point         */
-#define DSEM_METRICS_PER_ELEMENT 2  /* affine (parallelepiped) elements only     */
+/* Metric storage.  PER_POINT (the zero value, so the default) keeps the
+ * reference's metrics at every point and is the parity mode.  PER_ELEMENT
+ * keeps one value per element (parallelepiped elements only) and differs
+ * from the reference by its point-to-point rounding noise (about 3e-12
+ * relative L2 on the RHS at 22^3, p=8); AUTO picks PER_ELEMENT when every
+ * element is affine.  Both are opt-in. */
+#define DSEM_METRICS_PER_POINT   0  /* reference: metrics at every point (default) */
+#define DSEM_METRICS_AUTO        1  /* per element when every element is affine    */
+#define DSEM_METRICS_PER_ELEMENT 2  /* affine (parallelepiped) elements only       */
 
 typedef struct dsem_ctx dsem_ctx;
 
@@ -186,6 +192,15 @@ int  dsem_tape_reserve(dsem_ctx *ctx, int nslots);
 int  dsem_rk_step_record(dsem_ctx *ctx, double dt, int slot);
 int  dsem_rk_step_vjp_tape(dsem_ctx *ctx, double dt, int slot, double *dQbar, double *nubar);
 
+/* modalfilter (src/Energydissipation/energydissipation.f90:720-959; the
+ * driver calls it every 50 steps before tstep, src/3DDriver_parfmdf.f90:462-463)
+ * on the context's state: per element the momentum J*Q times jacob(1,1,1)
+ * through a 3-D DCT-II (FFTW REDFT10) with the reference's rescaling, modes
+ * with any index >= cutoff set to zero, and back through a 3-D DCT-III
+ * (REDFT01).  cutoff = p+1 keeps every mode, as the reference as shipped
+ * does (its mode edit is commented out): the identity up to round-off. */
+int  dsem_modal_filter(dsem_ctx *ctx, int cutoff);
+
 /* Device-to-device state transfer (checkpointing): dJQ [K][5][ncg^3]. */
 int  dsem_set_state_device(dsem_ctx *ctx, const double *dJQ);
 int  dsem_get_state_device(dsem_ctx *ctx, double *dJQ);
@@ -200,12 +215,24 @@ int  dsem_quadrature_weights(const dsem_ctx *ctx, double *bmat);
  * nu_t = Re (C_s Delta)^2 |S| from the lagged staggered states, capped at
  * numax, prolonged to the staggered grids; the viscous fluxes use
  * amu = 1 + nu_t rho (+ mu_h), Vis_Fluxes_new.f90:123-125.  Zero on the first
- * step after dsem_set_state.  The shock and density sensors are not built.
+ * step after dsem_set_state (sensor switches: dsem_set_les_sensors).
  * dsem_les_update recomputes nu_t from the current lagged states and returns
  * it at the Gauss points [K][ncg^3] and on the staggered grids
  * [K][3][nc*ncg^2] (either nullable); dsem_set_les_viscosity prescribes the
  * staggered field for dsem_rhs. */
 int  dsem_set_les(dsem_ctx *ctx, int enable, double cs, double numax);
+
+/* Sensor switches of the Smagorinsky model (input keywords shockSensor and
+ * rhoSensor, staticSmag.f90:68-69): nu_t is multiplied by 1 - ducros and/or
+ * by the density-based sensor (RHS.f90:263-275) before the clamp.  Both
+ * sensors are computed by the entropy-viscosity update of the same step
+ * (shock = 1, which tstep runs first, src/tstep.f90:36-43); without it they
+ * stay zero, as in the reference.  dsem_get_ev_sensors returns the Gauss-point
+ * sensors [K][ncg^3] of the last entropy-viscosity update (either nullable). */
+#define DSEM_LES_SHOCK_SENSOR 1
+#define DSEM_LES_RHO_SENSOR   2
+int  dsem_set_les_sensors(dsem_ctx *ctx, int sensors);
+int  dsem_get_ev_sensors(dsem_ctx *ctx, double *ducros, double *rho_sensor);
 int  dsem_les_update(dsem_ctx *ctx, double *nut_g, double *nut_s);
 int  dsem_set_les_viscosity(dsem_ctx *ctx, const double *nut_s);
 

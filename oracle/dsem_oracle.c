@@ -308,10 +308,19 @@ void or_euler_flux(const or_phys *ph, const double *Q, double *f, double *g, dou
     }
 }
 
-/* Roe flux with entropy fix: PhysLib.f90:231-320 */
-void or_riemann(const or_phys *ph, const double *nhat, double ds, const double *Ql,
-                const double *Qr, double *flux)
+/* Planted mistakes for the tests' controls (0 = none, the reference):
+ * OR_MUT_ENTROPY_FIX_SIGN flips the sign of the entropy-fix term hd1m/hd4,
+ * OR_MUT_SWAP_SIDES swaps the left/right states of interface_flux. */
+static int g_mutation = 0;
+
+void or_set_mutation(int m) { g_mutation = m; }
+
+/* Roe flux with entropy fix: PhysLib.f90:231-320.  eta (nullable) receives
+ * the entropy-fix eta1 (qtd >= 0, branch +1) or eta4 (qtd < 0, branch -1). */
+static void riemann_core(const or_phys *ph, const double *nhat, double ds, const double *Ql,
+                         const double *Qr, double *flux, double *eta, double *branch)
 {
+    const double fsgn = (g_mutation & OR_MUT_ENTROPY_FIX_SIGN) ? -1.0 : 1.0;
     const double gamma = ph->gamma;
     const double rho = Ql[0], rhou = Ql[1], rhov = Ql[2], rhow = Ql[3], rhoe = Ql[4];
     const double rhon = Qr[0], rhoun = Qr[1], rhovn = Qr[2], rhown = Qr[3], rhoen = Qr[4];
@@ -337,8 +346,9 @@ void or_riemann(const or_phys *ph, const double *nhat, double ds, const double *
         const double dw1 = 0.5 * ((pright - pleft) / atd2 - (qr - ql) * rtd / atd);
         const double sp1 = qtd - atd;
         const double sp1m = fmin(sp1, 0.0);
-        const double hd1m = ((gamma + 1.0) / 4.0 * atd / rtd) * dw1;
+        const double hd1m = fsgn * (((gamma + 1.0) / 4.0 * atd / rtd) * dw1);
         const double eta1 = fmax(-fabs(sp1) - hd1m, 0.0);
+        if (eta) { *eta = eta1; *branch = 1.0; }
         const double udw1 = dw1 * (sp1m - 0.5 * eta1);
         const double rql = rho * ql;
         flux[0] = ds * (rql + udw1);
@@ -350,8 +360,9 @@ void or_riemann(const or_phys *ph, const double *nhat, double ds, const double *
         const double dw4 = 0.5 * ((pright - pleft) / atd2 + (qr - ql) * rtd / atd);
         const double sp4 = qtd + atd;
         const double sp4p = fmax(sp4, 0.0);
-        const double hd4 = ((gamma + 1.0) / 4.0 * atd / rtd) * dw4;
+        const double hd4 = fsgn * (((gamma + 1.0) / 4.0 * atd / rtd) * dw4);
         const double eta4 = fmax(-fabs(sp4) + hd4, 0.0);
+        if (eta) { *eta = eta4; *branch = -1.0; }
         const double udw4 = dw4 * (sp4p + 0.5 * eta4);
         const double rqr = rhon * qr;
         flux[0] = ds * (rqr - udw4);
@@ -360,6 +371,12 @@ void or_riemann(const or_phys *ph, const double *nhat, double ds, const double *
         flux[3] = ds * (rqr * wr + pright * nhat[2] - udw4 * (wtd + atd * nhat[2]));
         flux[4] = ds * (rqr * hr - udw4 * (htd + qtd * atd));
     }
+}
+
+void or_riemann(const or_phys *ph, const double *nhat, double ds, const double *Ql,
+                const double *Qr, double *flux)
+{
+    riemann_core(ph, nhat, ds, Ql, Qr, flux, NULL, NULL);
 }
 
 /* x/y/z_vis_flux: PhysLib.f90:419-583 (each recomputes the same primitives). */
@@ -561,8 +578,13 @@ int or_rhs_les(const or_mesh *mesh, const or_phys *ph, const double *JQ, const d
                 }
                 const double *nh = mesh->nhat + ((size_t)mm * nf + q) * 3;
                 const double ds = mesh->norm[(size_t)mm * nf + q];
-                if (fm == 1 || fm == 3 || fm == 6) or_riemann(ph, nh, ds, Qs, Qm, fl);
-                else or_riemann(ph, nh, ds, Qm, Qs, fl);
+                /* left state = the minus side: the slave on master faces 1, 3, 6
+                 * (mortar_operations.f90:190-236) */
+                int slave_left = (fm == 1 || fm == 3 || fm == 6);
+                if (g_mutation & OR_MUT_SWAP_SIDES) slave_left = !slave_left;
+                double *eta = (dbg && dbg->eta) ? dbg->eta + ((size_t)mm * nf + q) * 2 : NULL;
+                if (slave_left) riemann_core(ph, nh, ds, Qs, Qm, fl, eta, eta ? eta + 1 : NULL);
+                else riemann_core(ph, nh, ds, Qm, Qs, fl, eta, eta ? eta + 1 : NULL);
                 for (int v = 0; v < NEQ; ++v) avg[v] = (Qm[v] + Qs[v]) * 0.5;
                 int grid, idx;
                 face_slot(fm, i, j, ncg, nc, &grid, &idx);
@@ -747,7 +769,7 @@ int or_rk_step(const or_mesh *mesh, const or_phys *ph, double *JQ, double *G, do
     memset(G, 0, sizeof(double) * n);
     double last = 0.0;
     for (int kk = 0; kk < 5; ++kk) {
-        or_debug dbg = {stag_out, NULL, NULL, NULL};
+        or_debug dbg = {stag_out, NULL, NULL, NULL, NULL};
         int rc = or_rhs(mesh, ph, JQ, muh, Qt, (kk == 4 && stag_out) ? &dbg : NULL, nthreads);
         if (rc) { free(Qt); return rc; }
         const double a = ark[kk], c = crk[kk];
@@ -869,6 +891,11 @@ static void ev_terms(const or_phys *ph, const double *Q, const double *Qx, const
 /* entropy_viscosity_smooting (EV_routines/Filter/viscosity_smoothing.f90:4-195):
  * in-place (Gauss-Seidel order) smoothing of one element's mu at the Gauss
  * points: corners, the six faces, then the interior. */
+static double jq_of(const double *JQ, int e, int v, int pt, int np)
+{
+    return JQ[((size_t)e * NEQ + v) * np + pt];
+}
+
 static void ev_smooth(double *mu, int ncg)
 {
 #define MU(n, m, l) mu[((n) - 1) + ncg * (((m) - 1) + ncg * ((l) - 1))]
@@ -933,8 +960,22 @@ static void ev_smooth(double *mu, int ncg)
  * + smoothing + EV_Prolong (Prolong_gtol.f90:7-51).  delta_g: staticSmag.f90:191-201.
  * The Strain routine called in between (EViscosity_GL.f90:314-348) stores a
  * field that nothing on this path reads, so it is not restated. */
+/* The density-based sensor of calculate_terms (RHS.f90:263-275), from the
+ * Gauss-point density gradient: sqrt of the sum over the three direction
+ * pairs of 1/(1/(r_a^2+eps) + 1/(r_b^2+eps) + 1), eps = 1e-9.  It feeds the
+ * Smagorinsky model's rhoSensor switch (staticSmag.f90:69). */
+static double rho_sensor(double rx, double ry, double rz)
+{
+    const double eps = 1.0e-9;
+    const double rhoX2 = rx * rx, rhoY2 = ry * ry, rhoZ2 = rz * rz;
+    return sqrt((1.0 / ((1.0 / (rhoX2 + eps)) + (1.0 / (rhoY2 + eps)) + 1.0)) +
+                (1.0 / ((1.0 / (rhoX2 + eps)) + (1.0 / (rhoZ2 + eps)) + 1.0)) +
+                (1.0 / ((1.0 / (rhoY2 + eps)) + (1.0 / (rhoZ2 + eps)) + 1.0)));
+}
+
 int or_ev_viscosity(const or_mesh *mesh, const or_phys *ph, const or_ev *ev, const double *JQ,
-                    const double *stag_lag, double *muhg, double *muh, double *resid, double *ducros)
+                    const double *stag_lag, double *muhg, double *muh, double *resid, double *ducros,
+                    double *rhosens)
 {
     ops_t o;
     if (make_ops(mesh->p, &o)) return -1;
@@ -961,6 +1002,19 @@ int or_ev_viscosity(const or_mesh *mesh, const or_phys *ph, const or_ev *ev, con
         const double *jq = JQ + (size_t)e * NEQ * np;
         const double *jac = mesh->jacob + (size_t)e * np;
         const double c3 = pow((double)(ncg * ncg * ncg), 1.0 / 3.0);
+        /* cmax_cap: the cap cmax * max(rho) * h * max(|u| + c) over the element
+         * (the commented-out line EViscosity_GL.f90:82 with its maxima from
+         * :44-67), with the physical pressure and sound speed c = sqrt(gamma p / rho) */
+        double rhomax = 0.0, utmax = 0.0;
+        if (ev->cmax_cap)
+            for (int pt = 0; pt < np; ++pt) {
+                double Q[NEQ];
+                for (int v = 0; v < NEQ; ++v) Q[v] = jq_of(JQ, e, v, pt, np) * mesh->jacob[(size_t)e * np + pt];
+                const double rho = Q[0], u = Q[1] / rho, v = Q[2] / rho, w = Q[3] / rho;
+                const double p = (gamma - 1.0) * (Q[4] - 0.5 * rho * (u * u + v * v + w * w));
+                rhomax = fmax(rhomax, rho);
+                utmax = fmax(utmax, sqrt(u * u + v * v + w * w) + sqrt(gamma * p / rho));
+            }
         for (int pt = 0; pt < np; ++pt) {
             double Q[NEQ], Qx[NEQ], Qy[NEQ], Qz[NEQ], T1[3], rhs, duc;
             for (int v = 0; v < NEQ; ++v) {
@@ -973,11 +1027,13 @@ int or_ev_viscosity(const or_mesh *mesh, const or_phys *ph, const or_ev *ev, con
             ev_terms(ph, Q, Qx, Qy, Qz, T1, &rhs, &duc);
             if (resid) resid[(size_t)e * np + pt] = rhs;
             if (ducros) ducros[(size_t)e * np + pt] = duc;
+            if (rhosens) rhosens[(size_t)e * np + pt] = rho_sensor(Qx[0], Qy[0], Qz[0]);
             const double rho = jq[0 * np + pt] * jac[pt];
             const double h = pow(fabs(1.0 / jac[pt]), 1.0 / 3.0) / c3;
             double mur = ev->cmu * ph->re * rho * (h * h) * fabs(rhs);
             if (ev->ducros) mur = mur * duc;
-            mu[pt] = fmin(fmax(mur, 0.0), ev->mumax);
+            const double cap = ev->cmax_cap ? ev->cmax * rhomax * h * utmax : ev->mumax;
+            mu[pt] = fmin(fmax(mur, 0.0), cap);
         }
         if (ev->smoothing) ev_smooth(mu, ncg);
         if (muhg)
@@ -1007,7 +1063,7 @@ int or_rk_step_ev(const or_mesh *mesh, const or_phys *ph, const or_ev *ev, doubl
     const int ns = (mesh->p + 2) * (mesh->p + 1) * (mesh->p + 1);
     const size_t n = (size_t)mesh->K * NEQ * np;
     if (have_lag) {
-        int rc = or_ev_viscosity(mesh, ph, ev, JQ, stag_lag, NULL, muh, NULL, NULL);
+        int rc = or_ev_viscosity(mesh, ph, ev, JQ, stag_lag, NULL, muh, NULL, NULL, NULL);
         if (rc) return rc;
     } else {
         memset(muh, 0, sizeof(double) * (size_t)mesh->K * 3 * ns);
@@ -1017,7 +1073,7 @@ int or_rk_step_ev(const or_mesh *mesh, const or_phys *ph, const or_ev *ev, doubl
     memset(G, 0, sizeof(double) * n);
     double last = 0.0;
     for (int kk = 0; kk < 5; ++kk) {
-        or_debug dbg = {kk == 4 ? stag_lag : NULL, NULL, NULL, NULL};
+        or_debug dbg = {kk == 4 ? stag_lag : NULL, NULL, NULL, NULL, NULL};
         int rc = or_rhs(mesh, ph, JQ, muh, Qt, kk == 4 ? &dbg : NULL, nthreads);
         if (rc) { free(Qt); return rc; }
         const double a = ark[kk], c = crk[kk];
@@ -1211,6 +1267,17 @@ int or_enstrophy(const or_mesh *mesh, const or_phys *ph, const double *JQ, const
 int or_smagorinsky(const or_mesh *mesh, const or_phys *ph, double cs, double numax, const double *JQ,
                    const double *stag_lag, double *nut_g, double *nut_s)
 {
+    return or_smagorinsky_sensors(mesh, ph, cs, numax, JQ, stag_lag, 0, NULL, NULL, nut_g, nut_s);
+}
+
+/* simpleSmag with its shockSensor (nu_t *= 1 - ducros) and rhoSensor
+ * (nu_t *= rhoSensor) switches, staticSmag.f90:67-69, before the clamp; the
+ * sensors are the Gauss-point fields that RHS_ent_trans left (nullable when
+ * the switch is off). */
+int or_smagorinsky_sensors(const or_mesh *mesh, const or_phys *ph, double cs, double numax, const double *JQ,
+                           const double *stag_lag, int sensors, const double *ducros, const double *rhosens,
+                           double *nut_g, double *nut_s)
+{
     ops_t o;
     if (make_ops(mesh->p, &o)) return -1;
     const int K = mesh->K, ncg = o.ncg, nc = o.nc, np = o.np, ns = o.ns;
@@ -1239,6 +1306,8 @@ int or_smagorinsky(const or_mesh *mesh, const or_phys *ph, double cs, double num
                                      + 2 * S23 * S23 + 2 * S31 * S31 + 2 * S32 * S32 + 2 * S33 * S33);
             const double delta = pow(fabs(1.0 / jac[pt]), 1.0 / 3.0) / c3;
             double nt = ph->re * (cs * delta) * (cs * delta) * Sbar;
+            if (sensors & OR_LES_SHOCK_SENSOR) nt = nt * (1.0 - ducros[(size_t)e * np + pt]);
+            if (sensors & OR_LES_RHO_SENSOR) nt = nt * rhosens[(size_t)e * np + pt];
             nu[pt] = fmin(fmax(nt, 0.0), numax);
         }
         if (nut_g)
@@ -1283,7 +1352,7 @@ int or_rk_step_les(const or_mesh *mesh, const or_phys *ph, double cs, double num
     memset(G, 0, sizeof(double) * n);
     double last = 0.0;
     for (int kk = 0; kk < 5; ++kk) {
-        or_debug dbg = {kk == 4 ? stag_lag : NULL, NULL, NULL, NULL};
+        or_debug dbg = {kk == 4 ? stag_lag : NULL, NULL, NULL, NULL, NULL};
         int rc = or_rhs_les(mesh, ph, JQ, NULL, nut_s, Qt, kk == 4 ? &dbg : NULL, nthreads);
         if (rc) { free(Qt); return rc; }
         const double a = ark[kk], c = crk[kk];
@@ -1300,5 +1369,95 @@ int or_rk_step_les(const or_mesh *mesh, const or_phys *ph, double cs, double num
     }
     if (resid) *resid = last;
     free(Qt);
+    return 0;
+}
+
+/* modalfilter (src/Energydissipation/energydissipation.f90:720-959) on the
+ * whole state, in place: per element and momentum field, in = Q jacob(1,1,1);
+ * mid = REDFT10 in all three directions (FFTW's unnormalised DCT-II,
+ * Y_k = 2 sum_j X_j cos(pi (j + 1/2) k / N)); each mode divided by 2N (index
+ * 1) or N (index >= 2) per direction in the reference's loop order; modes
+ * with any 0-based index >= cutoff zeroed (the commented-out edit at
+ * :778-783 with its 8,9 generalised; cutoff = N keeps all, as shipped);
+ * divided by 2 per direction for index >= 2; out = REDFT01 in all three
+ * directions (Y_k = X_0 + 2 sum_{j>=1} X_j cos(pi j (k + 1/2) / N));
+ * Q = out / jacob(1,1,1).  Direct O(N) sums (FFTW's fast algorithm is a
+ * different rounding of the same transform). */
+static void dct_line(const double *in, double *out, int N, int stride, int type)
+{
+    for (int k = 0; k < N; ++k) {
+        double s = 0.0;
+        for (int j = 0; j < N; ++j) {
+            double c;
+            if (type == 2) c = 2.0 * cos(PI_REF * (j + 0.5) * k / N);
+            else c = j == 0 ? 1.0 : 2.0 * cos(PI_REF * j * (k + 0.5) / N);
+            s += c * in[j * stride];
+        }
+        out[k * stride] = s;
+    }
+}
+
+static void dct3d(double *a, double *tmp, int N, int type)
+{
+    /* along n (fastest), then m, then l; a is [l][m][n] */
+    for (int d = 0; d < 3; ++d) {
+        const int stride = d == 0 ? 1 : (d == 1 ? N : N * N);
+        for (int x = 0; x < N; ++x)
+            for (int y = 0; y < N; ++y) {
+                int base;
+                if (d == 0) base = N * (x + N * y);
+                else if (d == 1) base = x + N * N * y;
+                else base = x + N * y;
+                double lin[9], lout[9];
+                for (int j = 0; j < N; ++j) lin[j] = a[base + j * stride];
+                dct_line(lin, lout, N, 1, type);
+                for (int j = 0; j < N; ++j) tmp[base + j * stride] = lout[j];
+            }
+        memcpy(a, tmp, sizeof(double) * N * N * N);
+    }
+}
+
+int or_modal_filter(const or_mesh *mesh, double *JQ, int cutoff)
+{
+    const int N = mesh->p + 1, np = N * N * N;
+    if (cutoff < 0 || cutoff > N) return -1;
+#pragma omp parallel for schedule(static)
+    for (int e = 0; e < mesh->K; ++e) {
+        const double j0 = mesh->jacob[(size_t)e * np];
+        for (int v = 1; v <= 3; ++v) {
+            double a[729], t[729];
+            double *q = JQ + ((size_t)e * NEQ + v) * np;
+            for (int i = 0; i < np; ++i) a[i] = q[i] * j0;
+            dct3d(a, t, N, 2);
+#define MID(i, j, k) a[((i) - 1) + N * (((j) - 1) + N * ((k) - 1))]
+            for (int i = 1; i <= N; ++i)
+                for (int j = 1; j <= N; ++j) {
+                    MID(1, i, j) = MID(1, i, j) / (2.0 * N);
+                    MID(i, 1, j) = MID(i, 1, j) / (2.0 * N);
+                    MID(i, j, 1) = MID(i, j, 1) / (2.0 * N);
+                }
+            for (int i = 1; i <= N; ++i)
+                for (int j = 1; j <= N; ++j)
+                    for (int k = 2; k <= N; ++k) {
+                        MID(i, j, k) = MID(i, j, k) / N;
+                        MID(j, k, i) = MID(j, k, i) / N;
+                        MID(k, i, j) = MID(k, i, j) / N;
+                    }
+            for (int i = 1; i <= N; ++i)
+                for (int j = 1; j <= N; ++j)
+                    for (int k = 1; k <= N; ++k)
+                        if (i > cutoff || j > cutoff || k > cutoff) MID(i, j, k) = 0.0;
+            for (int i = 1; i <= N; ++i)
+                for (int j = 1; j <= N; ++j)
+                    for (int k = 2; k <= N; ++k) {
+                        MID(i, j, k) = MID(i, j, k) / 2;
+                        MID(j, k, i) = MID(j, k, i) / 2;
+                        MID(k, i, j) = MID(k, i, j) / 2;
+                    }
+#undef MID
+            dct3d(a, t, N, 3);
+            for (int i = 0; i < np; ++i) q[i] = a[i] / j0;
+        }
+    }
     return 0;
 }

@@ -48,7 +48,17 @@ typedef struct {
     double *flux;   /* [K][3][5][ns] f,g,h after all face/viscous updates   */
     double *grad;   /* [K][3][5][np] Qx,Qy,Qz at Gauss points               */
     double *visc;   /* [K][3][5][ns] r,s,t                                   */
+    double *eta;    /* [M][nf][2] entropy-fix eta and branch (+1: qtd >= 0, -1) */
 } or_debug;
+
+/* modalfilter (energydissipation.f90:720-959) on JQ [K][5][np] in place;
+ * cutoff = p+1 keeps every mode (the reference as shipped). */
+int or_modal_filter(const or_mesh *mesh, double *JQ, int cutoff);
+
+/* Planted mistakes, for the tests' controls only (0 restores the reference). */
+#define OR_MUT_ENTROPY_FIX_SIGN 1
+#define OR_MUT_SWAP_SIDES       2
+void or_set_mutation(int m);
 
 int  or_ops(int p, double *cx, double *cxg, double *dmx, double *bx);
 void or_rkcoefs(double *ark, double *brk, double *crk);
@@ -70,6 +80,8 @@ int  or_rhs(const or_mesh *mesh, const or_phys *ph, const double *JQ, const doub
 typedef struct {
     double cmu, mumax;
     int32_t ducros, smoothing;
+    int32_t cmax_cap, pad_;   /* cap mu by cmax*max(rho)*h*max(|u|+c) instead of mumax */
+    double cmax;
 } or_ev;
 
 /* mu_h from the state JQ and the lagged staggered states stag_lag
@@ -77,7 +89,8 @@ typedef struct {
  * last stage).  Outputs: muhg [K][np] (Gauss points), muh [K][3][ns]
  * (staggered grids), resid/ducros [K][np] (nullable). */
 int  or_ev_viscosity(const or_mesh *mesh, const or_phys *ph, const or_ev *ev, const double *JQ,
-                     const double *stag_lag, double *muhg, double *muh, double *resid, double *ducros);
+                     const double *stag_lag, double *muhg, double *muh, double *resid, double *ducros,
+                     double *rhosens);
 
 /* tstep with entropy viscosity: mu_h from stag_lag (zero when have_lag = 0,
  * the reference's fresh-start behaviour), five stages, and stag_lag
@@ -93,6 +106,11 @@ int  or_rhs_les(const or_mesh *mesh, const or_phys *ph, const double *JQ, const 
                 const double *nut, double *Qt, or_debug *dbg, int nthreads);
 int  or_smagorinsky(const or_mesh *mesh, const or_phys *ph, double cs, double numax, const double *JQ,
                     const double *stag_lag, double *nut_g, double *nut_s);
+#define OR_LES_SHOCK_SENSOR 1
+#define OR_LES_RHO_SENSOR   2
+int  or_smagorinsky_sensors(const or_mesh *mesh, const or_phys *ph, double cs, double numax, const double *JQ,
+                            const double *stag_lag, int sensors, const double *ducros, const double *rhosens,
+                            double *nut_g, double *nut_s);
 int  or_rk_step_les(const or_mesh *mesh, const or_phys *ph, double cs, double numax, double *JQ, double *G,
                     double dt, double *stag_lag, int have_lag, double *nut_s, double *resid, int nthreads);
 

@@ -33,11 +33,12 @@ class OrPhys(C.Structure):
 
 
 class OrEv(C.Structure):
-    _fields_ = [("cmu", C.c_double), ("mumax", C.c_double), ("ducros", C.c_int32), ("smoothing", C.c_int32)]
+    _fields_ = [("cmu", C.c_double), ("mumax", C.c_double), ("ducros", C.c_int32), ("smoothing", C.c_int32),
+                ("cmax_cap", C.c_int32), ("pad_", C.c_int32), ("cmax", C.c_double)]
 
 
 class OrDebug(C.Structure):
-    _fields_ = [("stag", _dp), ("flux", _dp), ("grad", _dp), ("visc", _dp)]
+    _fields_ = [("stag", _dp), ("flux", _dp), ("grad", _dp), ("visc", _dp), ("eta", _dp)]
 
 
 def build(quiet: bool = True) -> None:
@@ -73,10 +74,14 @@ def lib():
                                  C.c_int]
         L.or_dtcalc.argtypes = [C.POINTER(OrMesh), C.POINTER(OrPhys), _dp]
         L.or_ev_viscosity.argtypes = [C.POINTER(OrMesh), C.POINTER(OrPhys), C.POINTER(OrEv), _dp, _dp, _dp, _dp,
-                                      _dp, _dp]
+                                      _dp, _dp, _dp]
         L.or_rk_step_ev.argtypes = [C.POINTER(OrMesh), C.POINTER(OrPhys), C.POINTER(OrEv), _dp, _dp, C.c_double,
                                     _dp, C.c_int, _dp, _dp, C.c_int]
         L.or_dtcalc.restype = C.c_double
+        L.or_set_mutation.argtypes = [C.c_int]
+        L.or_modal_filter.argtypes = [C.POINTER(OrMesh), _dp, C.c_int]
+        L.or_smagorinsky_sensors.argtypes = [C.POINTER(OrMesh), C.POINTER(OrPhys), C.c_double, C.c_double, _dp,
+                                             _dp, C.c_int, _dp, _dp, _dp, _dp]
         L.or_rhs_les.argtypes = [C.POINTER(OrMesh), C.POINTER(OrPhys), _dp, _dp, _dp, _dp, C.POINTER(OrDebug),
                                  C.c_int]
         L.or_smagorinsky.argtypes = [C.POINTER(OrMesh), C.POINTER(OrPhys), C.c_double, C.c_double, _dp, _dp, _dp,
@@ -189,8 +194,9 @@ def rhs(geom: Geometry, JQ: np.ndarray, ph: OrPhys | None = None, muh=None, debu
     if debug:
         K = geom.K
         out = dict(stag=np.zeros((K, 3, 5, geom.ns)), flux=np.zeros((K, 3, 5, geom.ns)),
-                   grad=np.zeros((K, 3, 5, geom.np)), visc=np.zeros((K, 3, 5, geom.ns)))
-        dbg = OrDebug(ptr(out["stag"]), ptr(out["flux"]), ptr(out["grad"]), ptr(out["visc"]))
+                   grad=np.zeros((K, 3, 5, geom.np)), visc=np.zeros((K, 3, 5, geom.ns)),
+                   eta=np.zeros((geom.M, geom.nf, 2)))
+        dbg = OrDebug(ptr(out["stag"]), ptr(out["flux"]), ptr(out["grad"]), ptr(out["visc"]), ptr(out["eta"]))
     mu = None if muh is None else np.ascontiguousarray(muh, dtype=np.float64)
     rc = lib().or_rhs(C.byref(m), C.byref(ph), ptr(JQ), ptr(mu), ptr(Qt),
                       C.byref(dbg) if dbg is not None else None, nthreads)
@@ -230,6 +236,35 @@ def euler_flux(Q, ph=None):
     return f, g, h
 
 
+def modal_filter(geom: Geometry, JQ, cutoff=None):
+    """modalfilter on a copy of JQ [K,5,np]; cutoff None keeps every mode."""
+    out = np.array(JQ, dtype=np.float64, order="C", copy=True)
+    m = geom.cmesh()
+    rc = lib().or_modal_filter(C.byref(m), ptr(out), geom.p + 1 if cutoff is None else int(cutoff))
+    assert rc == 0
+    return out
+
+
+MUT_ENTROPY_FIX_SIGN, MUT_SWAP_SIDES = 1, 2
+
+
+class mutation:
+    """Context manager planting a mistake in the oracle (tests' controls only):
+    MUT_ENTROPY_FIX_SIGN flips the sign of the Roe flux's entropy-fix term,
+    MUT_SWAP_SIDES swaps the left/right states of the interface flux."""
+
+    def __init__(self, m: int):
+        self.m = m
+
+    def __enter__(self):
+        lib().or_set_mutation(self.m)
+        return self
+
+    def __exit__(self, *exc):
+        lib().or_set_mutation(0)
+        return False
+
+
 def riemann(nhat, ds, Ql, Qr, ph=None):
     ph = ph or phys()
     out = np.zeros(5)
@@ -248,17 +283,20 @@ def vis_flux(Q, Qx, Qy, Qz, amu=1.0, ph=None):
 
 
 # ---------------------------------------------------------------- entropy viscosity
-def ev_viscosity(geom: Geometry, JQ, stag_lag, ev: OrEv, ph: OrPhys | None = None):
-    """(muhg [K,np], muh [K,3,ns], resid [K,np], ducros [K,np])."""
+def ev_viscosity(geom: Geometry, JQ, stag_lag, ev: OrEv, ph: OrPhys | None = None, rho_sensor=False):
+    """(muhg [K,np], muh [K,3,ns], resid [K,np], ducros [K,np]) and, with
+    rho_sensor, the density sensor [K,np] as a fifth entry."""
     ph = ph or phys()
     K = geom.K
     muhg, resid, duc = np.zeros((K, geom.np)), np.zeros((K, geom.np)), np.zeros((K, geom.np))
+    rs = np.zeros((K, geom.np))
     muh = np.zeros((K, 3, geom.ns))
     m = geom.cmesh()
     rc = lib().or_ev_viscosity(C.byref(m), C.byref(ph), C.byref(ev), ptr(np.ascontiguousarray(JQ)),
-                               ptr(np.ascontiguousarray(stag_lag)), ptr(muhg), ptr(muh), ptr(resid), ptr(duc))
+                               ptr(np.ascontiguousarray(stag_lag)), ptr(muhg), ptr(muh), ptr(resid), ptr(duc),
+                               ptr(rs))
     assert rc == 0
-    return muhg, muh, resid, duc
+    return (muhg, muh, resid, duc, rs) if rho_sensor else (muhg, muh, resid, duc)
 
 
 def rk_step_ev(geom: Geometry, JQ, dt, stag_lag, have_lag, ev: OrEv, ph: OrPhys | None = None, nthreads=0):
@@ -305,14 +343,22 @@ def enstrophy(geom: Geometry, JQ, stag_lag, ph: OrPhys | None = None) -> float:
 
 
 # ---------------------------------------------------------------- static Smagorinsky
-def smagorinsky(geom: Geometry, JQ, stag_lag, cs: float, numax: float, ph: OrPhys | None = None):
-    """(nu_t at the Gauss points [K,np], nu_t on the staggered grids [K,3,ns])."""
+LES_SHOCK_SENSOR, LES_RHO_SENSOR = 1, 2
+
+
+def smagorinsky(geom: Geometry, JQ, stag_lag, cs: float, numax: float, ph: OrPhys | None = None, sensors: int = 0,
+                ducros=None, rho_sensor=None):
+    """(nu_t at the Gauss points [K,np], nu_t on the staggered grids [K,3,ns]);
+    sensors: LES_SHOCK_SENSOR | LES_RHO_SENSOR with their Gauss-point fields."""
     ph = ph or phys()
     g = np.zeros((geom.K, geom.np))
     st = np.zeros((geom.K, 3, geom.ns))
     m = geom.cmesh()
-    assert lib().or_smagorinsky(C.byref(m), C.byref(ph), cs, numax, ptr(np.ascontiguousarray(JQ)),
-                                ptr(np.ascontiguousarray(stag_lag)), ptr(g), ptr(st)) == 0
+    d = None if ducros is None else np.ascontiguousarray(ducros, dtype=np.float64)
+    r = None if rho_sensor is None else np.ascontiguousarray(rho_sensor, dtype=np.float64)
+    assert lib().or_smagorinsky_sensors(C.byref(m), C.byref(ph), cs, numax, ptr(np.ascontiguousarray(JQ)),
+                                        ptr(np.ascontiguousarray(stag_lag)), int(sensors), ptr(d), ptr(r),
+                                        ptr(g), ptr(st)) == 0
     return g, st
 
 

@@ -20,7 +20,20 @@ def test_library_exports_every_declared_symbol():
 
 def test_timing_table_is_static():
     lib = L.load()
-    assert lib.dsem_timing_count() == 10
+    assert lib.dsem_timing_count() == 11
+
+
+def test_metric_default_is_the_reference_per_point_mode():
+    """A zero-initialised dsem_params selects per-point metrics (the parity
+    mode); per-element metrics are opt-in."""
+    import re
+    text = open(L.HEADER).read()
+    vals = dict(re.findall(r"#define (DSEM_METRICS_\w+)\s+(\d+)", text))
+    assert vals["DSEM_METRICS_PER_POINT"] == "0"
+    assert L.METRICS_PER_POINT == 0
+    import inspect
+    from dsem_amd.solver import DSEM
+    assert inspect.signature(DSEM.__init__).parameters["metrics"].default == "per_point"
 
 
 def _create(mesh, **kw):

@@ -8,7 +8,7 @@ import oracle as O
 from dsem_amd import cases
 from dsem_amd import mesh as M
 from dsem_amd.solver import DSEM
-from helpers import rel_l2
+from helpers import distorted_box, rel_l2
 
 pytestmark = pytest.mark.gpu
 
@@ -69,3 +69,52 @@ def test_ev_time_steps_match_oracle():
     assert _rel(muh, muh_r) < 1e-11
     assert np.abs(got - ref).max() / np.abs(ref).max() < 1e-13
     assert np.abs(lag - stag).max() / np.abs(stag).max() < 1e-13
+
+
+def test_entropy_step_p1_smoothing_in_reference_order():
+    """p = 1: every Gauss point is a corner and each neighbours three others,
+    so the in-place order of the corner updates decides the result."""
+    print("\n[case] ev update p=1 smoothing", flush=True)
+    mesh = distorted_box(3, 1, amp=0.05, seed=4)
+    g = O.Geometry(mesh)
+    JQ = cases.tgv_state(g.xg, g.jacob, "isothermal", perturb=2e-2, seed=5)
+    ph = O.phys(re=400.0)
+    stag = _lagged(mesh, g, JQ, ph)
+    ev = O.OrEv(0.8, 1e9, 0, 1)
+    mu_r, muh_r, _, _ = O.ev_viscosity(g, JQ, stag, ev, ph)
+    with DSEM(mesh, re=400.0, shock=True, cmu=0.8, mumax=1e9, smoothing=True) as s:
+        s.set_state(JQ)
+        s.set_lagged_states(stag)
+        mu, _, _ = s.ev_update()
+        _, muh = s.get_viscosity()
+    print("rel errors", _rel(mu, mu_r), _rel(muh, muh_r), flush=True)
+    assert _rel(mu, mu_r) < 1e-12 and _rel(muh, muh_r) < 1e-12
+
+
+@pytest.mark.parametrize("p", [4, 6])
+def test_cmax_cap_matches_oracle(p):
+    """The c_max cap (EViscosity_GL.f90:82, used by the isotropic-turbulence
+    case) against the oracle, on a mesh where it binds at some points only."""
+    print(f"\n[case] ev cmax cap p={p}", flush=True)
+    mesh = distorted_box(3, p, amp=0.05, seed=8)
+    g = O.Geometry(mesh)
+    JQ = cases.tgv_state(g.xg, g.jacob, "isothermal", perturb=3e-2, seed=7)
+    ph = O.phys(re=400.0)
+    stag = _lagged(mesh, g, JQ, ph)
+    ev = O.OrEv(30.0, 1e9, 1, 1)
+    ev.cmax_cap, ev.cmax = 1, 0.5
+    mu_r, muh_r, _, _ = O.ev_viscosity(g, JQ, stag, ev, ph)
+    ev0 = O.OrEv(30.0, 1e9, 1, 0)
+    ev0.cmax_cap, ev0.cmax = 1, 0.5
+    capped0, _, _, _ = O.ev_viscosity(g, JQ, stag, ev0, ph)
+    uncapped0, _, _, _ = O.ev_viscosity(g, JQ, stag, O.OrEv(30.0, 1e9, 1, 0), ph)
+    frac = np.mean(capped0 < uncapped0)
+    print(f"cap binds at {frac:.3f} of the points", flush=True)
+    assert 0.01 < frac < 0.5
+    with DSEM(mesh, re=400.0, shock=True, cmu=30.0, mumax=1e9, cmax=0.5, ducros=True, smoothing=True,
+              cmax_cap=True) as s:
+        s.set_state(JQ)
+        s.set_lagged_states(stag)
+        mu, _, _ = s.ev_update()
+        _, muh = s.get_viscosity()
+    assert _rel(mu, mu_r) < 1e-12 and _rel(muh, muh_r) < 1e-12

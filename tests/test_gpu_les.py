@@ -74,3 +74,31 @@ def test_les_steps_match_oracle():
         got = s.get_state()
     assert np.abs(nut_r).max() > 0
     assert np.abs(got - ref).max() / np.abs(ref).max() < 1e-13
+
+
+@pytest.mark.parametrize("sensors", [O.LES_SHOCK_SENSOR, O.LES_RHO_SENSOR, O.LES_SHOCK_SENSOR | O.LES_RHO_SENSOR])
+def test_sensor_switches_match_oracle(sensors):
+    """shockSensor / rhoSensor (staticSmag.f90:68-69): the sensors come from the
+    entropy-viscosity update of the same lagged states (tstep runs Entropy
+    before simpleSmag)."""
+    print(f"\n[case] smagorinsky sensors={sensors} distorted 3^3 p=5", flush=True)
+    mesh = distorted_box(3, 5, amp=0.06, seed=6)
+    g = O.Geometry(mesh)
+    JQ = cases.tgv_state(g.xg, g.jacob, "isothermal", perturb=2e-2, seed=5)
+    ph = O.phys(re=800.0)
+    _, dbg = O.rhs(g, JQ, ph, debug=True)
+    stag = np.ascontiguousarray(dbg["stag"])
+    _, _, _, duc_r, rs_r = O.ev_viscosity(g, JQ, stag, O.OrEv(1.0, 1.0, 0, 0), ph, rho_sensor=True)
+    ng_r, ns_r = O.smagorinsky(g, JQ, stag, 0.17, 1e9, ph, sensors, duc_r, rs_r)
+    with DSEM(mesh, re=800.0, shock=True, cmu=1.0, mumax=1.0) as s:
+        s.set_state(JQ)
+        s.set_les(True, 0.17, 1e9)
+        s.set_les_sensors(bool(sensors & O.LES_SHOCK_SENSOR), bool(sensors & O.LES_RHO_SENSOR))
+        s.set_lagged_states(stag)
+        s.ev_update()
+        duc, rs = s.ev_sensors()
+        ng, ns = s.les_update()
+    assert _rel(duc, duc_r) < 1e-12 and _rel(rs, rs_r) < 1e-12
+    assert _rel(ng, ng_r) < 1e-12 and _rel(ns, ns_r) < 1e-12
+    base, _ = O.smagorinsky(g, JQ, stag, 0.17, 1e9, ph)
+    assert _rel(ng, base) > 1e-3   # the switches act

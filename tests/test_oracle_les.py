@@ -63,3 +63,36 @@ def test_cap_and_rhs_coupling():
     b = O.rhs(g, JQ, ph, muh=nut * rho_stag)
     assert np.abs(a - b).max() <= 1e-12 * np.abs(b).max()
     assert np.abs(a - O.rhs(g, JQ, ph)).max() > 1e-6 * np.abs(b).max()   # the model acts
+
+
+def test_sensor_switches():
+    """shockSensor multiplies nu_t by 1 - ducros and rhoSensor by the density
+    sensor, before the clamp (staticSmag.f90:67-73).  Closed forms: a pure
+    shear has ducros = 0 (unchanged nu_t); an isotropic dilatation has
+    ducros = 9a^2/(9a^2 + 1e-9) (nu_t almost removed)."""
+    re, cs = 700.0, 0.17
+    shear = lambda x, y, z: (np.ones_like(x), 0.8 * y, 0 * x, 0 * x, P0 + 0 * x)
+    mesh, g, JQ, stag = _setup(shear)
+    _, _, _, duc, rs = O.ev_viscosity(g, JQ, stag, O.OrEv(1.0, 1.0, 0, 0), rho_sensor=True)
+    base, _ = O.smagorinsky(g, JQ, stag, cs, 1e9, O.phys(re=re))
+    sh, _ = O.smagorinsky(g, JQ, stag, cs, 1e9, O.phys(re=re), O.LES_SHOCK_SENSOR, duc, rs)
+    assert np.array_equal(sh, base)                      # ducros = 0 exactly for a shear
+    alpha = 0.3
+    dil = lambda x, y, z: (np.ones_like(x), alpha * x, alpha * y, alpha * z, P0 + 0 * x)
+    mesh, g, JQ, stag = _setup(dil)
+    _, _, _, duc, rs = O.ev_viscosity(g, JQ, stag, O.OrEv(1.0, 1.0, 0, 0), rho_sensor=True)
+    base, _ = O.smagorinsky(g, JQ, stag, cs, 1e9, O.phys(re=re))
+    sh, _ = O.smagorinsky(g, JQ, stag, cs, 1e9, O.phys(re=re), O.LES_SHOCK_SENSOR, duc, rs)
+    d9 = (3 * alpha) ** 2
+    assert np.allclose(sh, base * (1.0 - d9 / (d9 + 1e-9)), rtol=1e-6)
+    # both switches on a general field: the product of the two factors, then the clamp
+    fn = lambda x, y, z: (1.0 + 0.1 * np.sin(x), 1.3 * np.sin(y) * np.cos(z), 0.2 * np.cos(x), 0.1 * np.sin(z),
+                          P0 + np.cos(z))
+    mesh, g, JQ, stag = _setup(fn, n=2, p=5)
+    _, _, _, duc, rs = O.ev_viscosity(g, JQ, stag, O.OrEv(1.0, 1.0, 0, 0), rho_sensor=True)
+    base, _ = O.smagorinsky(g, JQ, stag, cs, 1e9, O.phys(re=re))
+    both, both_s = O.smagorinsky(g, JQ, stag, cs, 1e9, O.phys(re=re), O.LES_SHOCK_SENSOR | O.LES_RHO_SENSOR, duc, rs)
+    assert np.allclose(both, np.clip(base * (1.0 - duc) * rs, 0, 1e9), rtol=1e-13, atol=0)
+    assert np.ptp(rs) > 0 and np.ptp(duc) > 0
+    capped, _ = O.smagorinsky(g, JQ, stag, cs, 1e-7, O.phys(re=re), O.LES_RHO_SENSOR, duc, rs)
+    assert np.allclose(capped, np.clip(base * rs, 0, 1e-7), rtol=1e-13, atol=0)
