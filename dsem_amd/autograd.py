@@ -4,14 +4,18 @@
                dsem_rhs_device, backward dsem_rhs_vjp_device).
 ``RKSolve`` -- ``nsteps`` RK4(5) steps as a differentiable function of the
                initial state and of nu = 1/Re (forward dsem_rk_steps,
-               backward by step adjoints).  Checkpointing is two-level: the
-               forward pass keeps the state every ``every`` steps; the
-               backward pass re-runs one segment from its checkpoint while
-               recording each step's stage states and face intermediates on
-               the library's step tape (dsem_rk_step_record), then runs the
-               segment's step adjoints in reverse from the tape
-               (dsem_rk_step_vjp_tape).  Memory: nsteps/every checkpoints plus
-               ``every`` tape slots; work: one extra forward pass.
+               backward by step adjoints).  Checkpointing is two-level and
+               scheduled binomially (dsem_amd.checkpoint, revolve's optimum):
+               ``snaps`` state snapshots, and the library's step tape of
+               ``tape`` slots (dsem_rk_step_record stores a step's stage
+               states and face intermediates; dsem_rk_step_vjp_tape runs its
+               adjoint).  The forward pass keeps the snapshots the schedule
+               places on its way; the backward pass restores, advances,
+               snapshots and records/reverses as the schedule says.  The
+               default (tape = every, snaps = ceil(nsteps/every) - 1) is the
+               uniform schedule: a snapshot every ``every`` steps and one
+               extra forward pass; fewer slots trade memory for
+               recomputation at the binomial optimum.
 
 Tensors are FP64 on the solver's GPU, shaped [K, 5, ncg^3].  The library
 works on its own HIP stream and synchronises it before returning; this
@@ -56,42 +60,60 @@ class RHS(torch.autograd.Function):
 
 class RKSolve(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, JQ0: torch.Tensor, nu: torch.Tensor, solver: DSEM, nsteps: int, dt: float, every: int = 10):
+    def forward(ctx, JQ0: torch.Tensor, nu: torch.Tensor, solver: DSEM, nsteps: int, dt: float, every: int = 10,
+                snaps: int | None = None, tape: int | None = None):
+        from .checkpoint import Plan
         _check_state(solver, JQ0)
         re = solver.params["re"]
         if abs(float(nu) * re - 1.0) > 1e-12:
             raise ValueError("nu must equal 1/Re of the solver (the context's Re is fixed at creation)")
         every = max(1, int(every))
+        tape = max(1, min(int(tape if tape is not None else every), nsteps))
+        if snaps is None:
+            snaps = -(-nsteps // every) - 1
+        plan = Plan(nsteps, max(0, int(snaps)), tape)
+        keep = set(plan.forward_snapshots())
         torch.cuda.synchronize()
         solver.set_state_device(JQ0.data_ptr())
-        cps = [JQ0.detach().clone()]
+        held = {0: JQ0.detach().clone()}
         for s in range(1, nsteps + 1):
             solver.steps(1, dt, check=False)
-            if s % every == 0 and s < nsteps:
+            if s in keep:
                 t = torch.empty_like(JQ0)
                 solver.get_state_device(t.data_ptr())
-                cps.append(t)
+                held[s] = t
         out = torch.empty_like(JQ0)
         solver.get_state_device(out.data_ptr())
-        ctx.solver, ctx.nsteps, ctx.dt, ctx.every, ctx.cps = solver, nsteps, dt, every, cps
+        ctx.solver, ctx.dt, ctx.plan, ctx.held = solver, dt, plan, held
         return out
 
     @staticmethod
     def backward(ctx, gout: torch.Tensor):
-        solver, n, dt, every = ctx.solver, ctx.nsteps, ctx.dt, ctx.every
+        solver, dt, plan, held = ctx.solver, ctx.dt, ctx.plan, ctx.held
         qbar = gout.detach().to(torch.float64).contiguous().clone()
         torch.cuda.synchronize()
         nubar = 0.0
-        starts = list(range(0, n, every))
-        solver.tape_reserve(min(every, n))
-        for si in reversed(range(len(starts))):
-            s0, s1 = starts[si], min(starts[si] + every, n)
-            solver.set_state_device(ctx.cps[si].data_ptr())
-            for i in range(s1 - s0):
-                solver.rk_step_record(dt, i)
-            for i in reversed(range(s1 - s0)):
-                nubar += solver.rk_step_vjp_tape(dt, i, qbar.data_ptr())
-        return qbar, torch.tensor(nubar, dtype=torch.float64, device=qbar.device), None, None, None, None
+        solver.tape_reserve(plan.tape)
+        for act in plan.actions():
+            kind = act[0]
+            if kind == "restore":
+                solver.set_state_device(held[act[1]].data_ptr())
+            elif kind == "advance":
+                solver.steps(act[1], dt, check=False)
+            elif kind == "snapshot":
+                t = torch.empty_like(qbar)
+                solver.get_state_device(t.data_ptr())
+                held[act[1]] = t
+            elif kind == "free":
+                del held[act[1]]
+            else:   # reverse: record k steps from the current state, then their adjoints backwards
+                k = act[2]
+                for i in range(k):
+                    solver.rk_step_record(dt, i)
+                for i in reversed(range(k)):
+                    nubar += solver.rk_step_vjp_tape(dt, i, qbar.data_ptr())
+        ctx.held = None
+        return qbar, torch.tensor(nubar, dtype=torch.float64, device=qbar.device), None, None, None, None, None, None
 
 
 def kinetic_energy_integral(JQ: torch.Tensor, bmat: torch.Tensor) -> torch.Tensor:
@@ -101,10 +123,13 @@ def kinetic_energy_integral(JQ: torch.Tensor, bmat: torch.Tensor) -> torch.Tenso
 
 
 def solve(solver: DSEM, JQ0: torch.Tensor, nsteps: int, dt: float, every: int = 10,
-          nu: torch.Tensor | None = None) -> torch.Tensor:
+          nu: torch.Tensor | None = None, snaps: int | None = None, tape: int | None = None) -> torch.Tensor:
+    """``nsteps`` differentiable RK steps from JQ0.  Checkpointing: ``snaps``
+    snapshots and a ``tape``-slot step tape, scheduled binomially; by default
+    a snapshot every ``every`` steps (uniform)."""
     if nu is None:
         nu = torch.tensor(1.0 / solver.params["re"], dtype=torch.float64, device=JQ0.device)
-    return RKSolve.apply(JQ0, nu, solver, nsteps, dt, every)
+    return RKSolve.apply(JQ0, nu, solver, nsteps, dt, every, snaps, tape)
 
 
 def bmat_tensor(solver: DSEM, device) -> torch.Tensor:

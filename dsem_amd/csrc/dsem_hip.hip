@@ -145,8 +145,9 @@ struct Sz {
 struct DevMesh {
     int K, M, ms;                 // ms: 1 per-point metrics, 0 per-element
     const double *jacob, *gmetg, *gmet0, *gmet1, *gmet2;
-    const int *e2m;               // [K][6] (mortar << 1) | side
+    const int *e2m;               // [K][6] (mortar << 4) | (map << 1) | side, see e2m_mortar
     const int *m_elem, *m_face, *m_sign, *m_nsign;
+    const int *m_orient;          // [M] slave orientation 0..7, or null when every mortar is aligned
     const double *nhat, *norm;
 };
 
@@ -182,6 +183,30 @@ __device__ __forceinline__ int face_of(int d, int j)
     const bool hi = (j != 0);
     return d == 0 ? (hi ? 4 : 6) : (d == 1 ? (hi ? 2 : 1) : (hi ? 5 : 3));
 }
+
+// Face point maps of rotated conforming faces.  Orientation codes DFLT,
+// B1F2, B1B2, F1B2, F2F1, B2F1, B2B1, F2B1 = 0..7 (element_library.f90:99-101):
+// mortar point (I, J) takes the slave face's point orient_point(o, I + N J)
+// (rotate_to_mortar, mortar_operations.f90:461-605); rotate_flux_to_face
+// (:616-758) is the inverse map, which is the same code except that 5 and 7
+// exchange.  Written as an optional swap of (I, J) and a reflection of each.
+template <int N>
+__device__ __forceinline__ int orient_point(int o, int q)
+{
+    const int I = q % N, J = q / N;
+    const bool sw = o >= 4;
+    const bool fi = (o == 1) | (o == 2) | (o == 6) | (o == 7);
+    const bool fj = (o == 2) | (o == 3) | (o == 5) | (o == 6);
+    const int a = sw ? J : I, b = sw ? I : J;
+    return (fi ? N - 1 - a : a) + N * (fj ? N - 1 - b : b);
+}
+
+// e2m entry of an element face: its mortar, the side (0 master, 1 slave) and
+// the map code that takes the element's own face point to the mortar point
+// (0 on master sides and aligned mortars, the inverse orientation on
+// rotated slave faces).
+__device__ __forceinline__ int e2m_mortar(int c) { return c >> 4; }
+__device__ __forceinline__ int e2m_map(int c) { return (c >> 1) & 7; }
 
 // ------------------------------------------------------------ pointwise physics
 // Pointwise physics.  Euler fluxes: fflux/gflux/hflux (PhysLib.f90:17-124).
@@ -342,10 +367,11 @@ __global__ __launch_bounds__(BLOCK) void k_face_inv(const double *__restrict__ T
     const int em = m.m_elem[2 * mm], es = m.m_elem[2 * mm + 1];
     const int fm = m.m_face[2 * mm], fs = m.m_face[2 * mm + 1];
     double Qm[5], Qs[5], fl[5];
+    const int qs = m.m_orient ? orient_point<S::NCG>(m.m_orient[mm], q) : q;   // rotate_to_mortar
 #pragma unroll
     for (int v = 0; v < 5; ++v) {
         Qm[v] = T[(((size_t)em * 6 + fm - 1) * NEQ + v) * S::NF + q];
-        Qs[v] = T[(((size_t)es * 6 + fs - 1) * NEQ + v) * S::NF + q];
+        Qs[v] = T[(((size_t)es * 6 + fs - 1) * NEQ + v) * S::NF + qs];
     }
     const int nfm = m.ms ? S::NF : 1, qq = q * m.ms;
     const double n0 = m.nhat[((size_t)mm * 3 + 0) * nfm + qq];
@@ -489,8 +515,9 @@ __device__ __forceinline__ LaneFlux volume_grid(const DevMesh &m, const DevPhys 
             if (nut) mf_nut = nut[((size_t)e * 3 + D) * S::NS + mf_s];
             if (muh) mf_muh = muh[((size_t)e * 3 + D) * S::NS + mf_s];
             if (ml.j == 0 || ml.j == S::NC - 1) {
-                const int mo = e2m_e[face_of<P>(D, ml.j) - 1] >> 1;
-                const int q = a + S::NCG * b;
+                const int c = e2m_e[face_of<P>(D, ml.j) - 1];
+                const int mo = e2m_mortar(c);
+                const int q = orient_point<S::NCG>(e2m_map(c), a + S::NCG * b);
 #pragma unroll
                 for (int v = 0; v < 5; ++v) mf_qa[v] = QA[((size_t)mo * NEQ + v) * S::NF + q];
                 mf_nrm = m.norm[(size_t)mo * nfm + q * m.ms];
@@ -538,7 +565,8 @@ __device__ __forceinline__ LaneFlux volume_grid(const DevMesh &m, const DevPhys 
         const int s = MF ? mf_s : sidx<P>(D, jc, a, b);
         const bool face = (jc == 0 || jc == S::NC - 1);
         const int fc = face_of<P>(D, jc);
-        const int mo = e2m_e[fc - 1] >> 1;
+        const int mo = e2m_mortar(e2m_e[fc - 1]);
+        const int qm = orient_point<S::NCG>(e2m_map(e2m_e[fc - 1]), q);   // the face point on the mortar
         // Vector form: the point's metric row, and its line's face average and
         // norm (every lane: face_of names a face for any j, and interior lanes
         // hit the face lane's cache lines), requested before the contractions
@@ -548,8 +576,8 @@ __device__ __forceinline__ LaneFlux volume_grid(const DevMesh &m, const DevPhys 
 #pragma unroll
             for (int i = 0; i < 3; ++i) vg[i] = gmd[i * nsm + s * m.ms];
 #pragma unroll
-            for (int v = 0; v < 5; ++v) vqa[v] = QA[((size_t)mo * NEQ + v) * S::NF + q];
-            vnrm = m.norm[(size_t)mo * nfm + q * m.ms];
+            for (int v = 0; v < 5; ++v) vqa[v] = QA[((size_t)mo * NEQ + v) * S::NF + qm];
+            vnrm = m.norm[(size_t)mo * nfm + qm * m.ms];
         }
         double Qv[5];
         CT Bj[MF ? 1 : S::NCG];
@@ -741,9 +769,11 @@ typedef __attribute__((address_space(3))) void *LdsPtr;
 __device__ __forceinline__ void lds_dword(const void *src, void *lds_wave_base)
 {
     const unsigned l = __builtin_amdgcn_readfirstlane((unsigned)(size_t)(LdsPtr)lds_wave_base);
-    // M0 is declared clobbered, so the compiler keeps no live value of its
-    // own in it across this copy
-    asm volatile("s_mov_b32 m0, %1\n\ts_nop 0\n\tglobal_load_lds_dword %0, off" : : "v"(src), "s"(l) : "memory", "m0");
+    // The LDS base goes in through the {m0} operand constraint: the compiler
+    // writes M0 itself and knows the copy reads it, so no value of its own is
+    // kept there across the copy.  The s_nop separates that write from the
+    // copy's use of M0.
+    asm volatile("s_nop 0\n\tglobal_load_lds_dword %0, off" : : "v"(src), "{m0}"(l) : "memory");
 }
 
 // s_waitcnt immediate of gfx9 for vmcnt(0) alone (expcnt 7, lgkmcnt 15).
@@ -774,9 +804,12 @@ __device__ __forceinline__ void volume_inputs_lds(const double *__restrict__ JQ,
     const int lane = tid % 64;
     const int w = __builtin_amdgcn_readfirstlane(tid / 64);
     // mortars first, before any copy is in flight
-    int mo6[6];
+    int mo6[6], map6[6];
 #pragma unroll
-    for (int g = 0; g < 6; ++g) mo6[g] = e2m_e[g] >> 1;
+    for (int g = 0; g < 6; ++g) {
+        mo6[g] = e2m_mortar(e2m_e[g]);
+        map6[g] = e2m_map(e2m_e[g]);
+    }
     const char *q = (const char *)(JQ + (size_t)e * NQV);
     const char *jac = (const char *)(m.jacob + (size_t)e * (m.ms ? S::NP : 1));
 #pragma unroll
@@ -793,10 +826,14 @@ __device__ __forceinline__ void volume_inputs_lds(const double *__restrict__ JQ,
     for (int u = 0; u < (CA + NW - 1) / NW; ++u) {
         const int c = w + u * NW, d = c * 64 + lane;
         const int i = d >> 1, f = i / (NEQ * S::NF), vq = i % (NEQ * S::NF);
-        int mo = mo6[0];
+        int mo = mo6[0], mp = map6[0];
 #pragma unroll
-        for (int g = 1; g < 6; ++g) mo = f == g ? mo6[g] : mo;
-        if (d < DA) lds_dword((const char *)(QA + (size_t)mo * NEQ * S::NF + vq) + 4 * (d & 1), (char *)sA + 256 * c);
+        for (int g = 1; g < 6; ++g) {
+            mo = f == g ? mo6[g] : mo;
+            mp = f == g ? map6[g] : mp;
+        }
+        const int vqm = (vq / S::NF) * S::NF + orient_point<S::NCG>(mp, vq % S::NF);
+        if (d < DA) lds_dword((const char *)(QA + (size_t)mo * NEQ * S::NF + vqm) + 4 * (d & 1), (char *)sA + 256 * c);
     }
 }
 
@@ -816,9 +853,12 @@ __device__ __forceinline__ void volume_inputs(const double *__restrict__ JQ, con
     const double *q = JQ + (size_t)e * NEQ * S::NP;
     const double *jac = m.jacob + (size_t)e * (m.ms ? S::NP : 1);
     const int *e2m = m.e2m + (size_t)e * 6;
-    int mort[6];
+    int mort[6], map6[6];
 #pragma unroll
-    for (int f = 0; f < 6; ++f) mort[f] = e2m[f] >> 1;
+    for (int f = 0; f < 6; ++f) {
+        mort[f] = e2m_mortar(e2m[f]);
+        map6[f] = e2m_map(e2m[f]);
+    }
 #pragma unroll
     for (int u = 0; u < NQI; ++u) {
         const int i = tid + u * VBLOCK;
@@ -830,10 +870,14 @@ __device__ __forceinline__ void volume_inputs(const double *__restrict__ JQ, con
         const int i = tid + u * VBLOCK;
         const int f = i / (NEQ * S::NF), vq = i % (NEQ * S::NF);
         // mortar of face f+1 by selects (a dynamic index into mort[] would go to scratch)
-        int mo = mort[0];
+        int mo = mort[0], mp = map6[0];
 #pragma unroll
-        for (int g = 1; g < 6; ++g) mo = f == g ? mort[g] : mo;
-        av[u] = i < NAV ? QA[(size_t)mo * NEQ * S::NF + vq] : 0.0;
+        for (int g = 1; g < 6; ++g) {
+            mo = f == g ? mort[g] : mo;
+            mp = f == g ? map6[g] : mp;
+        }
+        const int vqm = (vq / S::NF) * S::NF + orient_point<S::NCG>(mp, vq % S::NF);
+        av[u] = i < NAV ? QA[(size_t)mo * NEQ * S::NF + vqm] : 0.0;
     }
 }
 
@@ -1170,10 +1214,11 @@ __global__ __launch_bounds__(BLOCK) void k_face_visc(const double *__restrict__ 
     const int nfm = m.ms ? S::NF : 1;
     const double nrm = m.norm[(size_t)mm * nfm + q * m.ms];
     const bool flip = m.m_nsign[mm] == -1;
+    const int qs = m.m_orient ? orient_point<S::NCG>(m.m_orient[mm], q) : q;   // rotate_fv_to_mortar
 #pragma unroll
     for (int v = 0; v < 5; ++v) {
         const double fvm = FV[(((size_t)em * 6 + fm - 1) * NEQ + v) * S::NF + q];
-        double fvs = FV[(((size_t)es * 6 + fs - 1) * NEQ + v) * S::NF + q];
+        double fvs = FV[(((size_t)es * 6 + fs - 1) * NEQ + v) * S::NF + qs];
         if (flip) fvs = -fvs;
         const double qb = 0.5 * (fvm + fvs) * nrm;
         FT[((size_t)mm * NEQ + v) * S::NF + q] = FI[((size_t)mm * NEQ + v) * S::NF + q] - qb / ph.re;
@@ -1242,13 +1287,14 @@ __global__ __launch_bounds__(UBLOCK) void k_update(const double *__restrict__ Qt
     // Mortar and orientation sign of the six faces: uniform per element, so
     // these are scalar loads; each work-item picks its face by selects.
     const int *e2m = m.e2m + (size_t)e * 6;
-    int fmo[6];
+    int fmo[6], fmp[6];
     double fsg[6];
 #pragma unroll
     for (int f = 0; f < 6; ++f) {
         const int c = e2m[f];
-        fmo[f] = c >> 1;
-        fsg[f] = ((c & 1) && m.m_sign[c >> 1] == -1) ? -1.0 : 1.0;
+        fmo[f] = e2m_mortar(c);
+        fmp[f] = e2m_map(c);
+        fsg[f] = ((c & 1) && m.m_sign[e2m_mortar(c)] == -1) ? -1.0 : 1.0;
     }
     // Every global load of the face fluxes and of the first batch of points
     // is issued before the first wait.
@@ -1257,10 +1303,14 @@ __global__ __launch_bounds__(UBLOCK) void k_update(const double *__restrict__ Qt
     for (int u = 0; u < NFI; ++u) {
         const int i = tid + u * UBLOCK;
         const int f = i / (NEQ * S::NF), r = i % (NEQ * S::NF);
-        int mo = fmo[0];
+        int mo = fmo[0], mp = fmp[0];
 #pragma unroll
-        for (int g = 1; g < 6; ++g) mo = f == g ? fmo[g] : mo;
-        fv[u] = i < NFT ? FT[(size_t)mo * NEQ * S::NF + r] : 0.0;
+        for (int g = 1; g < 6; ++g) {
+            mo = f == g ? fmo[g] : mo;
+            mp = f == g ? fmp[g] : mp;
+        }
+        const int rm = (r / S::NF) * S::NF + orient_point<S::NCG>(mp, r % S::NF);   // rotate_flux_to_face
+        fv[u] = i < NFT ? FT[(size_t)mo * NEQ * S::NF + rm] : 0.0;
     }
     const double *jac = m.jacob + (size_t)e * (m.ms ? S::NP : 1);
     const size_t e0 = (size_t)e * NQ;
@@ -1950,6 +2000,8 @@ struct dsem_ctx {
     double *tape = nullptr;
     int tape_slots = 0;
     int *e2m = nullptr, *m_elem = nullptr, *m_face = nullptr, *m_sign = nullptr, *m_nsign = nullptr;
+    int *m_orient = nullptr;                   // [M] orientation codes, allocated only when some face is rotated
+    bool rotated = false;                      // some mortar has orientation != 0
     unsigned long long *d_scalars = nullptr;   // [0] resid bits, [1] dt bits, [2] nonfinite flag
     std::vector<void *> allocations;
     // timing
@@ -2000,6 +2052,7 @@ DevMesh dev_mesh(const dsem_ctx *c)
     m.K = c->K; m.M = c->M; m.ms = c->ms;
     m.jacob = c->jacob; m.gmetg = c->gmetg; m.gmet0 = c->gmet[0]; m.gmet1 = c->gmet[1]; m.gmet2 = c->gmet[2];
     m.e2m = c->e2m; m.m_elem = c->m_elem; m.m_face = c->m_face; m.m_sign = c->m_sign; m.m_nsign = c->m_nsign;
+    m.m_orient = c->m_orient;
     m.nhat = c->nhat; m.norm = c->norm;
     return m;
 }
@@ -2072,9 +2125,19 @@ int volume_grid_size(dsem_ctx *c, int &grid)
 // traces_ready: T already holds the face traces of JQ (computed by the
 // previous stage's update); make_traces: this stage's update also computes
 // the traces of the updated state for the next stage.
+// Per-call intermediates of one RHS evaluation: face traces T, mortar
+// averages QA, Roe fluxes FI, viscous face traces FV, total face fluxes FT
+// and the partial residual Qtp.  Every entry a call reads is written earlier
+// in the same call, so none needs clearing.
+struct RhsBufs {
+    double *T, *QA, *FI, *FV, *FT, *Qtp;
+};
+
+RhsBufs ctx_bufs(const dsem_ctx *c) { return RhsBufs{c->T, c->QA, c->FI, c->FV, c->FT, c->Qtp}; }
+
 template <int P>
 int run_rhs(dsem_ctx *c, double *JQ, const StageArgs &st, double *Qt_out, bool traces_ready, bool make_traces,
-            bool save_lag)
+            bool save_lag, const RhsBufs &b)
 {
     using S = Sz<P>;
     const DevMesh m = dev_mesh(c);
@@ -2085,8 +2148,8 @@ int run_rhs(dsem_ctx *c, double *JQ, const StageArgs &st, double *Qt_out, bool t
     const double *nut = c->les ? c->nut : nullptr;
     hipStream_t s = c->stream;
     c->traces_cur = false;   // only dsem_rk_steps vouches for T after its last stage
-    if (!traces_ready) TRY(launch(c, KID_TRACES, [&] { k_traces<P><<<K, BLOCK, 0, s>>>(JQ, m, c->T); }));
-    TRY(launch(c, KID_FACE_INV, [&] { k_face_inv<P><<<face_blocks, BLOCK, 0, s>>>(c->T, m, ph, c->FI, c->QA); }));
+    if (!traces_ready) TRY(launch(c, KID_TRACES, [&] { k_traces<P><<<K, BLOCK, 0, s>>>(JQ, m, b.T); }));
+    TRY(launch(c, KID_FACE_INV, [&] { k_face_inv<P><<<face_blocks, BLOCK, 0, s>>>(b.T, m, ph, b.FI, b.QA); }));
     double *slag = save_lag ? c->Slag : nullptr;
     int gv0 = 0, gv1 = 0, gv2 = 0;
     if (c->params.precision == DSEM_PREC_FP32) TRY((volume_grid_size<P, 1>(c, gv1)));
@@ -2094,17 +2157,17 @@ int run_rhs(dsem_ctx *c, double *JQ, const StageArgs &st, double *Qt_out, bool t
     else TRY((volume_grid_size<P, 0>(c, gv0)));
     TRY(launch(c, KID_VOLUME, [&] {
         if (c->params.precision == DSEM_PREC_FP32)
-            k_volume<P, 1><<<gv1, vblock<P, 1>(), 0, s>>>(JQ, m, ph, c->QA, muh, nut, c->FV, c->Qtp, slag);
+            k_volume<P, 1><<<gv1, vblock<P, 1>(), 0, s>>>(JQ, m, ph, b.QA, muh, nut, b.FV, b.Qtp, slag);
         else if (c->params.precision == DSEM_PREC_BF16X3)
-            k_volume<P, 2><<<gv2, vblock<P, 2>(), 0, s>>>(JQ, m, ph, c->QA, muh, nut, c->FV, c->Qtp, slag);
+            k_volume<P, 2><<<gv2, vblock<P, 2>(), 0, s>>>(JQ, m, ph, b.QA, muh, nut, b.FV, b.Qtp, slag);
         else
-            k_volume<P, 0><<<gv0, vblock<P, 0>(), 0, s>>>(JQ, m, ph, c->QA, muh, nut, c->FV, c->Qtp, slag);
+            k_volume<P, 0><<<gv0, vblock<P, 0>(), 0, s>>>(JQ, m, ph, b.QA, muh, nut, b.FV, b.Qtp, slag);
     }));
-    TRY(launch(c, KID_FACE_VISC, [&] { k_face_visc<P><<<face_blocks, BLOCK, 0, s>>>(c->FV, c->FI, m, ph, c->FT); }));
+    TRY(launch(c, KID_FACE_VISC, [&] { k_face_visc<P><<<face_blocks, BLOCK, 0, s>>>(b.FV, b.FI, m, ph, b.FT); }));
     if (make_traces)
-        TRY(launch(c, KID_UPDATE, [&] { k_update<P, true><<<K, UBLOCK, 0, s>>>(c->Qtp, c->FT, m, st, JQ, c->G, Qt_out, c->T); }));
+        TRY(launch(c, KID_UPDATE, [&] { k_update<P, true><<<K, UBLOCK, 0, s>>>(b.Qtp, b.FT, m, st, JQ, c->G, Qt_out, b.T); }));
     else
-        TRY(launch(c, KID_UPDATE, [&] { k_update<P, false><<<K, UBLOCK, 0, s>>>(c->Qtp, c->FT, m, st, JQ, c->G, Qt_out, c->T); }));
+        TRY(launch(c, KID_UPDATE, [&] { k_update<P, false><<<K, UBLOCK, 0, s>>>(b.Qtp, b.FT, m, st, JQ, c->G, Qt_out, b.T); }));
     return DSEM_OK;
 }
 
@@ -2130,9 +2193,10 @@ int run_dtcalc(dsem_ctx *c, unsigned long long *out)
     }
 
 int rhs_dispatch(dsem_ctx *c, double *JQ, const StageArgs &st, double *Qt_out, bool traces_ready = false,
-                 bool make_traces = false, bool save_lag = false)
+                 bool make_traces = false, bool save_lag = false, const RhsBufs *bufs = nullptr)
 {
-    DISPATCH_P(c, run_rhs, JQ, st, Qt_out, traces_ready, make_traces, save_lag);
+    const RhsBufs b = bufs ? *bufs : ctx_bufs(c);
+    DISPATCH_P(c, run_rhs, JQ, st, Qt_out, traces_ready, make_traces, save_lag, b);
 }
 
 template <int P>
@@ -2279,7 +2343,7 @@ int run_rhs_vjp(dsem_ctx *c, double *JQ, FwdTape f, bool fwd, double *Gb, double
     const int face_blocks = (c->M * S::NF + BLOCK - 1) / BLOCK;
     if (fwd) {
         StageArgs st{0.0, 0.0, 0, 0, nullptr, nullptr};
-        TRY(run_rhs<P>(c, JQ, st, c->Qt, false, false, false));   // forward intermediates in c->T, QA, FV
+        TRY(run_rhs<P>(c, JQ, st, c->Qt, false, false, false, ctx_bufs(c)));   // forward intermediates in c->T, QA, FV
         f = FwdTape{c->T, c->QA, c->FV};
     }
     hipStream_t s = c->stream;
@@ -2451,15 +2515,22 @@ int create_impl(const dsem_mesh_desc *mesh, const dsem_params *prm, int device, 
     c->params = *prm;
     // topology checks: every element face in exactly one mortar, orientation 0
     std::vector<int> e2m((size_t)c->K * 6, -1);
-    std::vector<int> me(2 * (size_t)c->M), mf(2 * (size_t)c->M), msg(c->M, 1), mns(c->M, 1);
+    std::vector<int> me(2 * (size_t)c->M), mf(2 * (size_t)c->M), msg(c->M, 1), mns(c->M, 1), mor(c->M, 0);
+    if (c->M >= (1 << 27)) return set_err(c, DSEM_ERR_ARG, "too many mortars");
     for (int k = 0; k < c->M; ++k) {
-        if (mesh->mortar_orient && mesh->mortar_orient[k] != 0)
-            return set_err(c, DSEM_ERR_UNSUPPORTED, "rotated mortar faces (orientation != 0) are not supported");
+        const int o = mesh->mortar_orient ? mesh->mortar_orient[k] : 0;
+        if (o < 0 || o > 7) return set_err(c, DSEM_ERR_ARG, "mortar orientation must be 0..7");
+        mor[k] = o;
+        if (o) c->rotated = true;
         for (int s = 0; s < 2; ++s) {
             const int e = mesh->mortar_elem[2 * k + s], f = mesh->mortar_face[2 * k + s];
             if (e < 0 || e >= c->K || f < 1 || f > 6) return set_err(c, DSEM_ERR_ARG, "mortar element/face out of range");
             if (e2m[(size_t)e * 6 + f - 1] != -1) return set_err(c, DSEM_ERR_ARG, "element face in two mortars");
-            e2m[(size_t)e * 6 + f - 1] = (k << 1) | s;
+            // the slave reads mortar data through the inverse of its
+            // orientation (rotate_flux_to_face): the same code with 5 and 7
+            // exchanged
+            const int map = s == 0 ? 0 : (o == 5 ? 7 : (o == 7 ? 5 : o));
+            e2m[(size_t)e * 6 + f - 1] = (k << 4) | (map << 1) | s;
             me[2 * k + s] = e;
             mf[2 * k + s] = f;
         }
@@ -2512,6 +2583,7 @@ int create_impl(const dsem_mesh_desc *mesh, const dsem_params *prm, int device, 
     TRY(upload(c, &c->m_face, mf));
     TRY(upload(c, &c->m_sign, msg));
     TRY(upload(c, &c->m_nsign, mns));
+    if (c->rotated) TRY(upload(c, &c->m_orient, mor));
     const size_t nst = (size_t)c->K * NEQ * c->np;
     TRY(dev_alloc(c, &c->JQ, nst));
     TRY(dev_alloc(c, &c->G, nst));
@@ -2886,6 +2958,8 @@ static int vjp_check(dsem_ctx *c)
                        "the adjoint covers the path without entropy viscosity or Smagorinsky model");
     if (c->params.precision != DSEM_PREC_FP64)
         return set_err(c, DSEM_ERR_UNSUPPORTED, "the adjoint is built for the FP64 path");
+    if (c->rotated)
+        return set_err(c, DSEM_ERR_UNSUPPORTED, "the adjoint covers meshes whose conforming faces are all aligned");
     HIP_TRY(c, hipSetDevice(c->device));
     return adjoint_alloc(c);
 }

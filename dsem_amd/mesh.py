@@ -19,7 +19,14 @@ Reference behaviour restated here:
   x: face 4 of the x-max element master / face 6 slave -- the reference keeps
   the face-4 mortar there);
 * ``nsign`` -- ``set_mortars.f90:227-380`` (sign of the dot product of the two
-  faces' outward-coordinate normals, +1 on every aligned mesh).
+  faces' outward-coordinate normals, +1 on every aligned mesh);
+* orientation of the slave face against the mortar --
+  ``face_operations.f90:404-460`` (``faces_match``: the slave index and
+  direction along which the master face's corner 1->2 and 1->4 edges run,
+  mapped by ``orient_map``/``side_map`` of ``element_library.f90:99-145``
+  to DFLT, B1F2, B1B2, F1B2, F2F1, B2F1, B2B1, F2B1 = 0..7), and the
+  point map of ``rotate_to_mortar`` (``mortar_operations.f90:461-605``):
+  mortar point (I, J) takes the slave's face point ``slave_point(o, I, J)``.
 
 Face numbering (reference): 1 eta=0, 2 eta=1, 3 zeta=0, 4 xi=1, 5 zeta=1, 6 xi=0.
 Faces are passed to the library 1-based, elements 0-based.
@@ -52,6 +59,37 @@ _SIGN_VALS = [-1, 1, -1, 1, 1, -1,
 
 def sign_map(ifl: int, ifr: int) -> int:
     return _SIGN_VALS[(ifr - 1) * 6 + (ifl - 1)]
+
+
+# side_map(a, b) of element_library.f90 for face corners a, b = 1..4: the
+# face index (1 or 2) that runs from corner a to corner b, negative when it
+# runs backward; 0 where the corners are not neighbours.
+_SIDE_MAP = {(1, 2): 1, (2, 1): -1, (1, 4): 2, (4, 1): -2, (2, 3): 2, (3, 2): -2, (4, 3): 1, (3, 4): -1}
+# orient_map(is1, is2) of element_library.f90: DFLT, B1F2, B1B2, F1B2, F2F1,
+# B2F1, B2B1, F2B1 = 0..7
+_ORIENT_MAP = {(1, 2): 0, (-1, 2): 1, (-1, -2): 2, (1, -2): 3, (2, 1): 4, (-2, 1): 5, (-2, -1): 6, (2, -1): 7}
+
+
+def faces_match_orientation(target_nodes, test_nodes) -> int:
+    """Orientation of the test (slave) face against the target (master)
+    face from their corner node ids, as faces_match computes it
+    (face_operations.f90:404-460)."""
+    c = [list(test_nodes).index(t) + 1 for t in target_nodes]
+    is1 = _SIDE_MAP.get((c[0], c[1]), 0)
+    is2 = _SIDE_MAP.get((c[0], c[3]), 0)
+    o = _ORIENT_MAP.get((is1, is2))
+    if o is None:
+        raise ValueError("illegal orientation of element faces")
+    return o
+
+
+def slave_point(o: int, I, J, n: int):
+    """Slave face point (i, j) that mortar point (I, J) takes under
+    orientation o (rotate_to_mortar, mortar_operations.f90:461-605);
+    0-based indices on an n x n face."""
+    m = n - 1
+    return {0: (I, J), 1: (m - I, J), 2: (m - I, m - J), 3: (I, m - J),
+            4: (J, I), 5: (J, m - I), 6: (m - J, m - I), 7: (m - J, I)}[o]
 
 
 # (fixed axis, fixed value, in-face axis of mortar index i, of index j)
@@ -98,7 +136,7 @@ class Mesh:
     mortar_face: np.ndarray             # [M, 2] int32 (1..6)
     mortar_sign: np.ndarray             # [M] int32
     mortar_nsign: np.ndarray            # [M] int32
-    mortar_orient: np.ndarray = field(default=None)  # [M] int32, always 0 here
+    mortar_orient: np.ndarray = field(default=None)  # [M] int32, 0..7 (faces_match orientation)
 
     @property
     def K(self) -> int:
@@ -170,7 +208,7 @@ def build_topology(p: int, corners: np.ndarray, node_ids: np.ndarray | None = No
                 t = tuple(corners[e, a])
                 node_ids[e, a] = keys.setdefault(t, len(keys))
     face_owner: dict = {}
-    mortars = []  # [elem_m, face_m, elem_s, face_s, sign]
+    mortars = []  # [elem_m, face_m, elem_s, face_s, sign, orientation by node ids]
     for e in range(K):
         for f in range(1, 7):
             key = frozenset(int(node_ids[e, a]) for a in FACE_NODES[f])
@@ -180,9 +218,11 @@ def build_topology(p: int, corners: np.ndarray, node_ids: np.ndarray | None = No
                 mortars[mi][2] = e
                 mortars[mi][3] = f
                 mortars[mi][4] = sign_map(fm, f)
+                mortars[mi][5] = faces_match_orientation([int(node_ids[em, a]) for a in FACE_NODES[fm]],
+                                                         [int(node_ids[e, a]) for a in FACE_NODES[f]])
             else:
                 face_owner[key] = len(mortars)
-                mortars.append([e, f, -1, 0, 1])
+                mortars.append([e, f, -1, 0, 1, None])
     boundary = [m for m in mortars if m[2] < 0]
     mortars = [m for m in mortars if m[2] >= 0]
     if boundary:
@@ -204,9 +244,9 @@ def build_topology(p: int, corners: np.ndarray, node_ids: np.ndarray | None = No
                     raise ValueError(f"periodic pairing failed for element {ea} face {face_a}")
                 eb = cand[0]
                 if master_is_b:
-                    mortars.append([eb, face_b, ea, face_a, 1])
+                    mortars.append([eb, face_b, ea, face_a, 1, None])
                 else:
-                    mortars.append([ea, face_a, eb, face_b, 1])
+                    mortars.append([ea, face_a, eb, face_b, 1, None])
                 used.add((ea, face_a))
                 used.add((eb, face_b))
 
@@ -224,17 +264,30 @@ def build_topology(p: int, corners: np.ndarray, node_ids: np.ndarray | None = No
     mf = np.array([[m[1], m[3]] for m in mortars], dtype=np.int32).reshape(M, 2)
     ms = np.array([m[4] for m in mortars], dtype=np.int32)
     mn = np.ones(M, dtype=np.int32)
-    # conforming, orientation-0 check: parameter corners coincide (periodic
-    # pairs coincide after a shift along one axis)
+    mo = np.zeros(M, dtype=np.int32)
+    # Orientation from the geometry: the o under which the slave face's
+    # parameter corners land on the master's (periodic pairs after a shift
+    # along one axis).  Faces matched by node ids must agree with
+    # faces_match's orientation.
     for k in range(M):
-        cm = _face_param_corners(corners[me[k, 0]], int(mf[k, 0]))
-        cs = _face_param_corners(corners[me[k, 1]], int(mf[k, 1]))
-        dlt = cm - cs
-        if not np.allclose(dlt, dlt[0:1], rtol=0, atol=1e-9 * (1 + np.abs(cm).max())):
-            raise ValueError(f"mortar {k}: faces are not aligned with orientation 0 (rotated faces are outside this path)")
+        cm = _face_param_corners(corners[me[k, 0]], int(mf[k, 0])).reshape(2, 2, 3)   # [J][I]
+        cs = _face_param_corners(corners[me[k, 1]], int(mf[k, 1])).reshape(2, 2, 3)
+        tol = 1e-9 * (1 + np.abs(cm).max())
+        found = []
+        for o in range(8):
+            mapped = np.array([[cs[slave_point(o, I, J, 2)[1], slave_point(o, I, J, 2)[0]] for I in (0, 1)]
+                               for J in (0, 1)])
+            dlt = (cm - mapped).reshape(4, 3)
+            if np.allclose(dlt, dlt[0:1], rtol=0, atol=tol):
+                found.append(o)
+        if len(found) != 1:
+            raise ValueError(f"mortar {k}: the two faces are not conforming")
+        mo[k] = found[0]
+        if mortars[k][5] is not None and mortars[k][5] != mo[k]:
+            raise ValueError(f"mortar {k}: node-id orientation {mortars[k][5]} != geometric orientation {mo[k]}")
         mn[k] = _nsign(corners[me[k, 0]], int(mf[k, 0]), corners[me[k, 1]], int(mf[k, 1]))
     mesh = Mesh(p=p, corners=corners, mortar_elem=me, mortar_face=mf, mortar_sign=ms,
-                mortar_nsign=mn, mortar_orient=np.zeros(M, dtype=np.int32))
+                mortar_nsign=mn, mortar_orient=mo)
     mesh.check()
     return mesh
 
@@ -275,6 +328,61 @@ def periodic_box(n: int, p: int, length: float = 2.0 * math.pi, rounded: bool = 
                 for a, (i, j, k) in enumerate(cn):
                     corners[e, a] = (x[i], x[j], x[k])
                     nodes[e, a] = nid(i, j, k)
+                e += 1
+    return build_topology(p, corners, nodes, periodic=True)
+
+
+_REF_VERTS = np.array([(0, 0, 0), (1, 0, 0), (1, 1, 0), (0, 1, 0), (0, 0, 1), (1, 0, 1), (1, 1, 1), (0, 1, 1)])
+
+
+def cube_rotations() -> list[np.ndarray]:
+    """The 24 orientation-preserving symmetries of the unit cube, each as the
+    corner relabelling perm with new corner a = old corner perm[a]."""
+    import itertools
+    out = []
+    for axes in itertools.permutations(range(3)):
+        for flips in itertools.product((0, 1), repeat=3):
+            A = np.zeros((3, 3))
+            for r, (a, f) in enumerate(zip(axes, flips)):
+                A[r, a] = -1.0 if f else 1.0
+            if np.linalg.det(A) < 0:
+                continue
+            # old reference point of new corner a: X_old = A X_new + shift
+            shift = np.array([1.0 if f else 0.0 for f in flips])
+            perm = []
+            for v in _REF_VERTS:
+                xo = A @ v + shift
+                perm.append(int(np.nonzero(np.all(_REF_VERTS == xo, axis=1))[0][0]))
+            out.append(np.array(perm))
+    return out
+
+
+def rotated_box(n: int, p: int, seed: int = 0, length: float = 2.0 * math.pi) -> Mesh:
+    """Periodic n^3 box whose interior elements (those not on the periodic
+    boundary, n >= 3) have their local frames turned by seeded random cube
+    symmetries: the same physical mesh as periodic_box with conforming faces
+    of every orientation 0..7 between neighbours."""
+    if n < 3:
+        raise ValueError("n >= 3: the boundary elements stay aligned for the periodic pairing")
+    base = periodic_box(n, p, length)
+    rots = cube_rotations()
+    rng = np.random.default_rng(seed)
+    x = box_nodes(n, length)
+    N1 = n + 1
+    corners = base.corners.copy()
+    nodes = np.zeros((n ** 3, 8), dtype=np.int64)
+    e = 0
+    for iz in range(n):
+        for ix in range(n):
+            for iy in range(n):
+                ids = [iy + N1 * (ix + N1 * iz), iy + N1 * (ix + 1 + N1 * iz), iy + 1 + N1 * (ix + 1 + N1 * iz),
+                       iy + 1 + N1 * (ix + N1 * iz), iy + N1 * (ix + N1 * (iz + 1)), iy + N1 * (ix + 1 + N1 * (iz + 1)),
+                       iy + 1 + N1 * (ix + 1 + N1 * (iz + 1)), iy + 1 + N1 * (ix + N1 * (iz + 1))]
+                nodes[e] = ids
+                if 0 < ix < n - 1 and 0 < iy < n - 1 and 0 < iz < n - 1:
+                    perm = rots[int(rng.integers(len(rots)))]
+                    corners[e] = base.corners[e][perm]
+                    nodes[e] = np.array(ids)[perm]
                 e += 1
     return build_topology(p, corners, nodes, periodic=True)
 

@@ -46,9 +46,11 @@ class DSEM:
                       np.ascontiguousarray(mesh.mortar_face, dtype=np.int32),
                       np.ascontiguousarray(mesh.mortar_sign, dtype=np.int32),
                       np.ascontiguousarray(mesh.mortar_nsign, dtype=np.int32),
-                      np.ascontiguousarray(mesh.corners, dtype=np.float64)]
-        me, mf, ms, mn, cr = self._keep
-        desc = L.MeshDesc(mesh.K, mesh.p, mesh.nmortar, 0, _ip(me), _ip(mf), _ip(ms), _ip(mn), None, _dp(cr))
+                      np.ascontiguousarray(mesh.corners, dtype=np.float64),
+                      np.ascontiguousarray(np.zeros(mesh.nmortar) if mesh.mortar_orient is None else mesh.mortar_orient,
+                                           dtype=np.int32)]
+        me, mf, ms, mn, cr, mo = self._keep
+        desc = L.MeshDesc(mesh.K, mesh.p, mesh.nmortar, 0, _ip(me), _ip(mf), _ip(ms), _ip(mn), _ip(mo), _dp(cr))
         flags = (L.EV_DUCROS if ducros else 0) | (L.EV_SMOOTHING if smoothing else 0) | \
             (L.EV_CMAX_CAP if cmax_cap else 0)
         prm = L.Params(gamma, re, pr, mach, int(bool(shock)), _PREC[precision], _METRICS[metrics], flags,

@@ -73,8 +73,13 @@ typedef struct dsem_ctx dsem_ctx;
 /* Conforming hex8 mesh.  Mortar side 0 is the master face (the one whose
  * geometry defines the normal, as in set_domain_geometry.f90:254-345).
  * Faces use the reference numbering 1..6 (1: eta=0, 2: eta=1, 3: zeta=0,
- * 4: xi=1, 5: zeta=1, 6: xi=0).  Only orientation 0 (aligned faces) is
- * supported. */
+ * 4: xi=1, 5: zeta=1, 6: xi=0).  mortar_orient is the orientation of the
+ * slave face against the mortar as faces_match returns it
+ * (src/Grid_Routines/face_operations.f90:404-460): DFLT, B1F2, B1B2, F1B2,
+ * F2F1, B2F1, B2B1, F2B1 = 0..7 (element_library.f90:99-101); mortar point
+ * (i,j) takes the slave face point that rotate_to_mortar
+ * (src/Parabolic3D/mortar_operations.f90:461-605) maps to it.  The adjoint
+ * entry points need every orientation to be 0. */
 typedef struct {
     int32_t K;                     /* elements                               */
     int32_t p;                     /* order: ncg = p+1 Gauss, nc = p+2 Lobatto, 1..8 */
@@ -84,7 +89,7 @@ typedef struct {
     const int32_t *mortar_face;    /* [nmortar][2] 1..6                       */
     const int32_t *mortar_sign;    /* [nmortar] +-1 (slave flux sign) or NULL */
     const int32_t *mortar_nsign;   /* [nmortar] +-1 (normal sign) or NULL     */
-    const int32_t *mortar_orient;  /* [nmortar] must be 0, or NULL            */
+    const int32_t *mortar_orient;  /* [nmortar] 0..7, or NULL (all 0)         */
     const double  *corners;        /* [K][8][3] hex8 corners, FE order        */
 } dsem_mesh_desc;
 
@@ -245,6 +250,7 @@ int  dsem_keep_lagged_states(dsem_ctx *ctx, int enable);
 /* Device-pointer entry points (same HIP device, stream-ordered on the
  * context's stream): dJQ and dQt are [K][5][ncg^3] FP64 device buffers. */
 int  dsem_rhs_device(dsem_ctx *ctx, const double *dJQ, double *dQt);
+
 int  dsem_state_pointers(dsem_ctx *ctx, double **dJQ, double **dG);
 int  dsem_stream(dsem_ctx *ctx, void **hip_stream);
 int  dsem_synchronize(dsem_ctx *ctx);

@@ -438,6 +438,27 @@ static void face_slot(int face, int i, int j, int ncg, int nc, int *grid, int *i
     }
 }
 
+/* Slave face point (i,j) that mortar point (I,J) takes under orientation o:
+ * rotate_to_mortar (mortar_operations.f90:461-605); its inverse is
+ * rotate_flux_to_face (:616-758) and rotate_fv_to_mortar (:851-995) is the
+ * same map for the viscous traces.  o = DFLT, B1F2, B1B2, F1B2, F2F1, B2F1,
+ * B2B1, F2B1 = 0..7 (element_library.f90:99-101). */
+static void slave_point(int o, int I, int J, int n, int *i, int *j)
+{
+    const int m = n - 1;
+    if (g_mutation & OR_MUT_ORIENT_INVERSE) o = (o == 5) ? 7 : (o == 7 ? 5 : o);
+    switch (o) {
+    case 1: *i = m - I; *j = J; break;
+    case 2: *i = m - I; *j = m - J; break;
+    case 3: *i = I; *j = m - J; break;
+    case 4: *i = J; *j = I; break;
+    case 5: *i = J; *j = m - I; break;
+    case 6: *i = m - J; *j = m - I; break;
+    case 7: *i = m - J; *j = I; break;
+    default: *i = I; *j = J; break;
+    }
+}
+
 /* Staggered index of Lobatto position j on the line (a,b) of grid d. */
 static inline int sidx(int d, int j, int a, int b, int ncg, int nc)
 {
@@ -546,17 +567,19 @@ int or_rhs_les(const or_mesh *mesh, const or_phys *ph, const double *JQ, const d
         }
     }
 
-    /* ---------------- send_q_to_mortars ---------------- */
+    /* ---------------- send_q_to_mortars (+ rotate_to_mortar) ---------------- */
 #pragma omp parallel for schedule(static)
     for (int mm = 0; mm < mesh->nmort; ++mm) {
         for (int side = 0; side < 2; ++side) {
             const int e = mesh->m_elem[2 * mm + side], f = mesh->m_face[2 * mm + side];
-            for (int j = 0; j < ncg; ++j)
-                for (int i = 0; i < ncg; ++i) {
-                    int grid, idx;
+            const int o = (side == 1 && mesh->m_orient) ? mesh->m_orient[mm] : 0;
+            for (int J = 0; J < ncg; ++J)
+                for (int I = 0; I < ncg; ++I) {
+                    int grid, idx, i, j;
+                    slave_point(o, I, J, ncg, &i, &j);
                     face_slot(f, i, j, ncg, nc, &grid, &idx);
                     for (int v = 0; v < NEQ; ++v)
-                        mQ[(((size_t)mm * 2 + side) * NEQ + v) * nf + i + ncg * j] =
+                        mQ[(((size_t)mm * 2 + side) * NEQ + v) * nf + I + ncg * J] =
                             S[(((size_t)e * 3 + grid) * NEQ + v) * ns + idx];
                 }
         }
@@ -592,7 +615,10 @@ int or_rhs_les(const or_mesh *mesh, const or_phys *ph, const double *JQ, const d
                     F[(((size_t)em * 3 + grid) * NEQ + v) * ns + idx] = fl[v];
                     S[(((size_t)em * 3 + grid) * NEQ + v) * ns + idx] = avg[v];
                 }
-                face_slot(fs, i, j, ncg, nc, &grid, &idx);
+                /* rotate_flux_to_face: the slave's face point of this mortar point */
+                int is, js;
+                slave_point(mesh->m_orient ? mesh->m_orient[mm] : 0, i, j, ncg, &is, &js);
+                face_slot(fs, is, js, ncg, nc, &grid, &idx);
                 for (int v = 0; v < NEQ; ++v) {
                     F[(((size_t)es * 3 + grid) * NEQ + v) * ns + idx] = sgn * fl[v];
                     S[(((size_t)es * 3 + grid) * NEQ + v) * ns + idx] = avg[v];
@@ -669,18 +695,20 @@ int or_rhs_les(const or_mesh *mesh, const or_phys *ph, const double *JQ, const d
         }
     }
 
-    /* ---------------- send_flux_to_mortars: fv = r/norm ---------------- */
+    /* ---------------- send_flux_to_mortars: fv = r/norm (+ rotate_fv_to_mortar) ---------------- */
 #pragma omp parallel for schedule(static)
     for (int mm = 0; mm < mesh->nmort; ++mm) {
         for (int side = 0; side < 2; ++side) {
             const int e = mesh->m_elem[2 * mm + side], f = mesh->m_face[2 * mm + side];
-            for (int j = 0; j < ncg; ++j)
-                for (int i = 0; i < ncg; ++i) {
-                    int grid, idx;
+            const int o = (side == 1 && mesh->m_orient) ? mesh->m_orient[mm] : 0;
+            for (int J = 0; J < ncg; ++J)
+                for (int I = 0; I < ncg; ++I) {
+                    int grid, idx, i, j;
+                    slave_point(o, I, J, ncg, &i, &j);
                     face_slot(f, i, j, ncg, nc, &grid, &idx);
-                    const double nrm = mesh->norm[(size_t)mm * nf + i + ncg * j];
+                    const double nrm = mesh->norm[(size_t)mm * nf + I + ncg * J];
                     for (int v = 0; v < NEQ; ++v)
-                        mFV[(((size_t)mm * 2 + side) * NEQ + v) * nf + i + ncg * j] =
+                        mFV[(((size_t)mm * 2 + side) * NEQ + v) * nf + I + ncg * J] =
                             R[(((size_t)e * 3 + grid) * NEQ + v) * ns + idx] / nrm;
                 }
         }
@@ -708,7 +736,9 @@ int or_rhs_les(const or_mesh *mesh, const or_phys *ph, const double *JQ, const d
                     double *t = &F[(((size_t)em * 3 + grid) * NEQ + v) * ns + idx];
                     *t = *t - Qb[v] / re;
                 }
-                face_slot(fs, i, j, ncg, nc, &grid, &idx);
+                int is, js;
+                slave_point(mesh->m_orient ? mesh->m_orient[mm] : 0, i, j, ncg, &is, &js);
+                face_slot(fs, is, js, ncg, nc, &grid, &idx);
                 for (int v = 0; v < NEQ; ++v) {
                     const double fl = (mesh->m_sign[mm] == -1) ? -Qb[v] : Qb[v];
                     double *t = &F[(((size_t)es * 3 + grid) * NEQ + v) * ns + idx];

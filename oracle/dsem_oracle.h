@@ -36,6 +36,7 @@ typedef struct {
     const double *jacob, *gmetg, *gmetx, *gmety, *gmetz;
     const int32_t *m_elem, *m_face, *m_sign, *m_nsign;
     const double *nhat, *norm;
+    const int32_t *m_orient;   /* [M] 0..7 (faces_match orientation of the slave face), or NULL: all 0 */
 } or_mesh;
 
 typedef struct {
@@ -58,6 +59,7 @@ int or_modal_filter(const or_mesh *mesh, double *JQ, int cutoff);
 /* Planted mistakes, for the tests' controls only (0 restores the reference). */
 #define OR_MUT_ENTROPY_FIX_SIGN 1
 #define OR_MUT_SWAP_SIDES       2
+#define OR_MUT_ORIENT_INVERSE   4   /* B2F1 and F2B1 read with each other's map (the inverse) */
 void or_set_mutation(int m);
 
 int  or_ops(int p, double *cx, double *cxg, double *dmx, double *bx);

@@ -25,7 +25,7 @@ class OrMesh(C.Structure):
     _fields_ = [("K", C.c_int32), ("p", C.c_int32), ("nmort", C.c_int32), ("pad_", C.c_int32),
                 ("jacob", _dp), ("gmetg", _dp), ("gmetx", _dp), ("gmety", _dp), ("gmetz", _dp),
                 ("m_elem", _ip), ("m_face", _ip), ("m_sign", _ip), ("m_nsign", _ip),
-                ("nhat", _dp), ("norm", _dp)]
+                ("nhat", _dp), ("norm", _dp), ("m_orient", _ip)]
 
 
 class OrPhys(C.Structure):
@@ -170,6 +170,8 @@ class Geometry:
         self.m_face = np.ascontiguousarray(mesh.mortar_face, dtype=np.int32)
         self.m_sign = np.ascontiguousarray(mesh.mortar_sign, dtype=np.int32)
         self.m_nsign = np.ascontiguousarray(mesh.mortar_nsign, dtype=np.int32)
+        orient = getattr(mesh, "mortar_orient", None)
+        self.m_orient = np.ascontiguousarray(np.zeros(M, dtype=np.int32) if orient is None else orient, dtype=np.int32)
         self.nhat = np.zeros((M, nf, 3))
         self.norm = np.zeros((M, nf))
         rc = lib().or_mortar_geometry(M, p, ptr(self.corners), ptr(self.m_elem), ptr(self.m_face),
@@ -181,7 +183,7 @@ class Geometry:
     def cmesh(self) -> OrMesh:
         return OrMesh(self.K, self.p, self.M, 0, ptr(self.jacob), ptr(self.gmetg), ptr(self.gmet[0]),
                       ptr(self.gmet[1]), ptr(self.gmet[2]), ptr(self.m_elem), ptr(self.m_face),
-                      ptr(self.m_sign), ptr(self.m_nsign), ptr(self.nhat), ptr(self.norm))
+                      ptr(self.m_sign), ptr(self.m_nsign), ptr(self.nhat), ptr(self.norm), ptr(self.m_orient))
 
 
 def rhs(geom: Geometry, JQ: np.ndarray, ph: OrPhys | None = None, muh=None, debug=False, nthreads=0):
@@ -245,13 +247,15 @@ def modal_filter(geom: Geometry, JQ, cutoff=None):
     return out
 
 
-MUT_ENTROPY_FIX_SIGN, MUT_SWAP_SIDES = 1, 2
+MUT_ENTROPY_FIX_SIGN, MUT_SWAP_SIDES, MUT_ORIENT_INVERSE = 1, 2, 4
 
 
 class mutation:
     """Context manager planting a mistake in the oracle (tests' controls only):
     MUT_ENTROPY_FIX_SIGN flips the sign of the Roe flux's entropy-fix term,
-    MUT_SWAP_SIDES swaps the left/right states of the interface flux."""
+    MUT_SWAP_SIDES swaps the left/right states of the interface flux,
+    MUT_ORIENT_INVERSE reads B2F1 and F2B1 slave faces with each other's
+    point map (the inverse map)."""
 
     def __init__(self, m: int):
         self.m = m

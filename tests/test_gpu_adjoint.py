@@ -216,3 +216,34 @@ def test_taylor_green_adjoint_gradient_8x8x8_p6():
           r["nu"], flush=True)
     assert r["velocity_directions"]["max_rel_err"] < 1e-6
     assert r["nu"]["rel_err"] < 1e-6
+
+
+def test_binomial_schedules_give_the_uniform_gradient():
+    """Memory-constrained binomial schedules (few snapshots, short tape)
+    recompute states by plain steps; their gradients equal those of the
+    uniform schedule (snapshot and tape of `every` steps)."""
+    import torch
+    from dsem_amd import autograd as A
+    from dsem_amd.checkpoint import Plan
+    print("\n[case] binomial checkpoint schedules 2^3 p=4, 12 steps", flush=True)
+    mesh = M.periodic_box(2, 4)
+    g, JQ = _state(mesh, 7, perturb=1e-2)
+    dt = 0.4 * O.dtcalc(g, JQ, O.phys(re=800.0))
+    n = 12
+    grads = {}
+    with DSEM(mesh, re=800.0, metrics="per_point") as s:
+        bmat = A.bmat_tensor(s, "cuda")
+        for key in [(None, None), (2, 1), (1, 3), (0, 2), (3, 2)]:
+            snaps, tape = key
+            x = torch.tensor(JQ, device="cuda", requires_grad=True)
+            nu = torch.tensor(1.0 / 800.0, dtype=torch.float64, device="cuda", requires_grad=True)
+            J = A.kinetic_energy_integral(A.solve(s, x, n, dt, every=4, nu=nu, snaps=snaps, tape=tape), bmat)
+            J.backward()
+            grads[key] = (x.grad.cpu().numpy(), float(nu.grad))
+            if snaps is not None:
+                print(f"snaps {snaps} tape {tape}: {Plan(n, snaps, tape).recomputed_steps()} recomputed steps",
+                      flush=True)
+    ref_x, ref_nu = grads[(None, None)]
+    for key, (gx, gnu) in grads.items():
+        assert np.linalg.norm(gx - ref_x) <= 1e-13 * np.linalg.norm(ref_x), key
+        assert abs(gnu - ref_nu) <= 1e-12 * abs(ref_nu), key

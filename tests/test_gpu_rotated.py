@@ -1,0 +1,78 @@
+"""GPU RHS and RK steps on meshes with rotated conforming faces (all eight
+orientations, rotate_to_mortar / rotate_flux_to_face / rotate_fv_to_mortar,
+mortar_operations.f90:461-995) against the oracle, whose maps are pinned by
+the geometry and by frame invariance (tests/test_oracle_rotated.py); the
+adjoint refuses such meshes."""
+import numpy as np
+import pytest
+
+import oracle as O
+from dsem_amd import _lib as L
+from dsem_amd import cases
+from dsem_amd import mesh as M
+from dsem_amd.solver import DSEM
+from helpers import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("p,seed,metrics", [(3, 0, "per_point"), (5, 1, "per_point"), (8, 2, "per_point"),
+                                            (7, 3, "per_point"), (4, 2, "per_element")])
+def test_rotated_faces_rhs_matches_oracle(p, seed, metrics):
+    print(f"\n[case] rotated faces rhs p={p} seed={seed} {metrics}", flush=True)
+    mesh = M.rotated_box(4, p, seed=seed)
+    assert len(set(mesh.mortar_orient.tolist())) == 8
+    g = O.Geometry(mesh)
+    JQ = cases.tgv_state(g.xg, g.jacob, "isothermal", perturb=1e-2, seed=seed)
+    want = O.rhs(g, JQ, O.phys(re=400.0))
+    with DSEM(mesh, re=400.0, metrics=metrics) as s:
+        s.set_state(JQ)
+        got = s.rhs()
+    err = rel_l2(got, want)
+    print("rel L2 per field", err, flush=True)
+    assert np.all(err <= (1e-12 if metrics == "per_point" else 5e-12))
+
+
+def test_rotated_faces_rk_steps_match_oracle():
+    print("\n[case] rotated faces 3 RK steps p=6", flush=True)
+    mesh = M.rotated_box(4, 6, seed=5)
+    g = O.Geometry(mesh)
+    JQ = cases.tgv_state(g.xg, g.jacob, "isothermal", perturb=1e-2, seed=2)
+    ph = O.phys(re=400.0)
+    dt = 0.5 * O.dtcalc(g, JQ, ph)
+    ref = JQ.copy()
+    for _ in range(3):
+        _, _, r_ref = O.rk_step(g, ref, dt, ph)
+    with DSEM(mesh, re=400.0) as s:
+        s.set_state(JQ)
+        for _ in range(3):
+            r = s.tstep(0.0, dt)
+        got = s.get_state()
+    err = rel_l2(got - JQ, ref - JQ)
+    print("state change rel L2 per field", err, "resid", r, r_ref, flush=True)
+    assert np.all(err <= 1e-12)
+    assert abs(r - r_ref) <= 1e-10 * abs(r_ref)
+
+
+def test_adjoint_refuses_rotated_faces():
+    mesh = M.rotated_box(3, 3, seed=0)
+    with DSEM(mesh) as s:
+        s.set_state(np.ones((mesh.K, 5, 64)))
+        with pytest.raises(L.DsemError) as ei:
+            s.rhs_vjp(np.ones((mesh.K, 5, 64)))
+        assert ei.value.code == L.DSEM_ERR_UNSUPPORTED
+
+
+@pytest.mark.parametrize("precision", ["fp32", "bf16x3"])
+def test_rotated_faces_mixed_precision_rhs(precision):
+    print(f"\n[case] rotated faces rhs p=6 {precision}", flush=True)
+    mesh = M.rotated_box(4, 6, seed=4)
+    g = O.Geometry(mesh)
+    JQ = cases.tgv_state(g.xg, g.jacob, "isothermal", perturb=1e-2, seed=9)
+    want = O.rhs(g, JQ, O.phys(re=400.0))
+    with DSEM(mesh, re=400.0, precision=precision) as s:
+        s.set_state(JQ)
+        got = s.rhs()
+    err = rel_l2(got, want)
+    print("rel L2 per field", err, flush=True)
+    assert np.all(err <= (1e-5 if precision == "fp32" else 1e-3))
