@@ -144,6 +144,7 @@ struct Sz {
 
 struct DevMesh {
     int K, M, ms;                 // ms: 1 per-point metrics, 0 per-element
+    int rot;                      // some mortar face is rotated (orientation != 0)
     const double *jacob, *gmetg, *gmet0, *gmet1, *gmet2;
     const int *e2m;               // [K][6] (mortar << 4) | (map << 1) | side, see e2m_mortar
     const int *m_elem, *m_face, *m_sign, *m_nsign;
@@ -517,7 +518,7 @@ __device__ __forceinline__ LaneFlux volume_grid(const DevMesh &m, const DevPhys 
             if (ml.j == 0 || ml.j == S::NC - 1) {
                 const int c = e2m_e[face_of<P>(D, ml.j) - 1];
                 const int mo = e2m_mortar(c);
-                const int q = orient_point<S::NCG>(e2m_map(c), a + S::NCG * b);
+                const int q = m.rot ? orient_point<S::NCG>(e2m_map(c), a + S::NCG * b) : a + S::NCG * b;
 #pragma unroll
                 for (int v = 0; v < 5; ++v) mf_qa[v] = QA[((size_t)mo * NEQ + v) * S::NF + q];
                 mf_nrm = m.norm[(size_t)mo * nfm + q * m.ms];
@@ -566,7 +567,7 @@ __device__ __forceinline__ LaneFlux volume_grid(const DevMesh &m, const DevPhys 
         const bool face = (jc == 0 || jc == S::NC - 1);
         const int fc = face_of<P>(D, jc);
         const int mo = e2m_mortar(e2m_e[fc - 1]);
-        const int qm = orient_point<S::NCG>(e2m_map(e2m_e[fc - 1]), q);   // the face point on the mortar
+        const int qm = m.rot ? orient_point<S::NCG>(e2m_map(e2m_e[fc - 1]), q) : q;   // the face point on the mortar
         // Vector form: the point's metric row, and its line's face average and
         // norm (every lane: face_of names a face for any j, and interior lanes
         // hit the face lane's cache lines), requested before the contractions
@@ -832,7 +833,7 @@ __device__ __forceinline__ void volume_inputs_lds(const double *__restrict__ JQ,
             mo = f == g ? mo6[g] : mo;
             mp = f == g ? map6[g] : mp;
         }
-        const int vqm = (vq / S::NF) * S::NF + orient_point<S::NCG>(mp, vq % S::NF);
+        const int vqm = m.rot ? (vq / S::NF) * S::NF + orient_point<S::NCG>(mp, vq % S::NF) : vq;
         if (d < DA) lds_dword((const char *)(QA + (size_t)mo * NEQ * S::NF + vqm) + 4 * (d & 1), (char *)sA + 256 * c);
     }
 }
@@ -876,7 +877,7 @@ __device__ __forceinline__ void volume_inputs(const double *__restrict__ JQ, con
             mo = f == g ? mort[g] : mo;
             mp = f == g ? map6[g] : mp;
         }
-        const int vqm = (vq / S::NF) * S::NF + orient_point<S::NCG>(mp, vq % S::NF);
+        const int vqm = m.rot ? (vq / S::NF) * S::NF + orient_point<S::NCG>(mp, vq % S::NF) : vq;
         av[u] = i < NAV ? QA[(size_t)mo * NEQ * S::NF + vqm] : 0.0;
     }
 }
@@ -1309,7 +1310,7 @@ __global__ __launch_bounds__(UBLOCK) void k_update(const double *__restrict__ Qt
             mo = f == g ? fmo[g] : mo;
             mp = f == g ? fmp[g] : mp;
         }
-        const int rm = (r / S::NF) * S::NF + orient_point<S::NCG>(mp, r % S::NF);   // rotate_flux_to_face
+        const int rm = m.rot ? (r / S::NF) * S::NF + orient_point<S::NCG>(mp, r % S::NF) : r;   // rotate_flux_to_face
         fv[u] = i < NFT ? FT[(size_t)mo * NEQ * S::NF + rm] : 0.0;
     }
     const double *jac = m.jacob + (size_t)e * (m.ms ? S::NP : 1);
@@ -2053,6 +2054,7 @@ DevMesh dev_mesh(const dsem_ctx *c)
     m.jacob = c->jacob; m.gmetg = c->gmetg; m.gmet0 = c->gmet[0]; m.gmet1 = c->gmet[1]; m.gmet2 = c->gmet[2];
     m.e2m = c->e2m; m.m_elem = c->m_elem; m.m_face = c->m_face; m.m_sign = c->m_sign; m.m_nsign = c->m_nsign;
     m.m_orient = c->m_orient;
+    m.rot = c->rotated ? 1 : 0;
     m.nhat = c->nhat; m.norm = c->norm;
     return m;
 }
