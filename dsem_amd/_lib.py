@@ -110,6 +110,7 @@ def load() -> C.CDLL:
         "dsem_rk_steps": ([vp, C.c_double, C.c_double, C.c_int, _dp], C.c_int),
         "dsem_dtcalc": ([vp, _dp], C.c_int),
         "dsem_rhs_device": ([vp, vp, vp], C.c_int),
+        "dsem_rhs_device_noalloc": ([vp, vp, vp], C.c_int),
         "dsem_state_pointers": ([vp, C.POINTER(vp), C.POINTER(vp)], C.c_int),
         "dsem_stream": ([vp, C.POINTER(vp)], C.c_int),
         "dsem_synchronize": ([vp], C.c_int),

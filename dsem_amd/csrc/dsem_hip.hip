@@ -40,6 +40,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -74,6 +75,12 @@ __constant__ double c_Dint[9][9];  // sum over the interior Lobatto j of D[k][j]
 // C3[k*N + j] = (j == 0 ? 1 : 2 cos(pi j (k + 1/2) / N))
 __constant__ double c_dct2[9][81];
 __constant__ double c_dct3[9][81];
+
+// Intermediates of dsem_rhs_device_noalloc: one copy per GPU, held until the
+// process ends.  Host code uses the device address that hipGetSymbolAddress
+// gives for the current device (kept per device by scratch_setup), never the
+// name itself.
+__device__ __align__(256) double scratch[DSEM_NOALLOC_SCRATCH_BYTES / sizeof(double)];
 
 // Contraction arithmetic of the volume kernel, by precision mode:
 //   PM 0  FP64 (the drop-in path);
@@ -2004,6 +2011,7 @@ struct dsem_ctx {
     int *m_orient = nullptr;                   // [M] orientation codes, allocated only when some face is rotated
     bool rotated = false;                      // some mortar has orientation != 0
     unsigned long long *d_scalars = nullptr;   // [0] resid bits, [1] dt bits, [2] nonfinite flag
+    double *scratch = nullptr;                 // device address of this device's `scratch` array
     std::vector<void *> allocations;
     // timing
     bool timing = false;
@@ -2500,6 +2508,26 @@ int upload_operators(dsem_ctx *c)
     return DSEM_OK;
 }
 
+// Device address of `scratch` on the context's device, looked up once per
+// device with that device current and shared by every context on it.
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double *g_scratch_dev[kMaxDevices] = {};
+
+int scratch_setup(dsem_ctx *c)
+{
+    if (c->device < 0 || c->device >= kMaxDevices) return set_err(c, DSEM_ERR_ARG, "device index out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_dev[c->device]) {
+        void *ptr = nullptr;
+        HIP_TRY(c, hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        if (!ptr) return set_err(c, DSEM_ERR_HIP, "hipGetSymbolAddress(scratch) returned no address");
+        g_scratch_dev[c->device] = static_cast<double *>(ptr);
+    }
+    c->scratch = g_scratch_dev[c->device];
+    return DSEM_OK;
+}
+
 int create_impl(const dsem_mesh_desc *mesh, const dsem_params *prm, int device, dsem_ctx *c)
 {
     if (!mesh || !prm) return set_err(c, DSEM_ERR_ARG, "null mesh or params");
@@ -2546,6 +2574,7 @@ int create_impl(const dsem_mesh_desc *mesh, const dsem_params *prm, int device, 
 
     HIP_TRY(c, hipSetDevice(device));
     HIP_TRY(c, hipStreamCreate(&c->stream));
+    TRY(scratch_setup(c));
     HIP_TRY(c, hipEventCreate(&c->ev0));
     HIP_TRY(c, hipEventCreate(&c->ev1));
     TRY(upload_operators(c));
@@ -2887,6 +2916,36 @@ int dsem_rhs_device(dsem_ctx *c, const double *dJQ, double *dQt)
     HIP_TRY(c, hipSetDevice(c->device));
     StageArgs st{0.0, 0.0, 0, 0, nullptr, nullptr};
     return rhs_dispatch(c, const_cast<double *>(dJQ), st, dQt);
+}
+
+int dsem_rhs_device_noalloc(dsem_ctx *c, const double *dJQ, double *dQt)
+{
+    if (!c || !c->stream || !dJQ || !dQt) return set_err(c, DSEM_ERR_ARG, "rhs_device_noalloc arguments");
+    if (!c->scratch) return set_err(c, DSEM_ERR_ARG, "no scratch address for this device");
+    // T, FV [K][6][5][nf]; QA, FI, FT [M][5][nf]; Qtp [K][5][np]; each part
+    // starts on a 256-byte boundary
+    size_t used = 0;
+    auto take = [&](size_t n) {
+        double *p = c->scratch + used;
+        used += (n + 31) / 32 * 32;
+        return p;
+    };
+    const size_t nface = (size_t)c->K * 6 * NEQ * c->nf, nmort = (size_t)c->M * NEQ * c->nf;
+    RhsBufs b;
+    b.T = take(nface);
+    b.FV = take(nface);
+    b.QA = take(nmort);
+    b.FI = take(nmort);
+    b.FT = take(nmort);
+    b.Qtp = take((size_t)c->K * NEQ * c->np);
+    if (used * sizeof(double) > DSEM_NOALLOC_SCRATCH_BYTES)
+        return set_err(c, DSEM_ERR_ARG, "rhs_device_noalloc: this mesh needs " + std::to_string(used * sizeof(double)) +
+                                            " bytes of scratch, more than the " +
+                                            std::to_string((unsigned long long)DSEM_NOALLOC_SCRATCH_BYTES) +
+                                            " bytes of the static array (use dsem_rhs_device)");
+    HIP_TRY(c, hipSetDevice(c->device));
+    StageArgs st{0.0, 0.0, 0, 0, nullptr, nullptr};
+    return rhs_dispatch(c, const_cast<double *>(dJQ), st, dQt, false, false, false, &b);
 }
 
 int dsem_rhs(dsem_ctx *c, double t, double *Qt_out)

@@ -251,6 +251,21 @@ int  dsem_keep_lagged_states(dsem_ctx *ctx, int enable);
  * context's stream): dJQ and dQt are [K][5][ncg^3] FP64 device buffers. */
 int  dsem_rhs_device(dsem_ctx *ctx, const double *dJQ, double *dQt);
 
+/* dsem_rhs_device without any allocation.  No counterpart in the reference:
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * kernels and result as dsem_rhs_device, but the RHS intermediates (face
+ * traces, mortar averages and fluxes, viscous face traces, partial residual:
+ * 8 * (12 K ncg^2 * 5 + 15 nmortar ncg^2 + 5 K ncg^3) bytes, each part rounded
+ * up to 256 bytes) live in one static device array of
+ * DSEM_NOALLOC_SCRATCH_BYTES per GPU, held until the process ends, instead of
+ * the context's buffers.  Meshes that need more return DSEM_ERR_ARG (at p = 8
+ * the limit is about 690 elements).  Every call on a GPU shares that GPU's
+ * array: calls that overlap on one GPU (contexts on different streams, or
+ * threads) race; calls on one context are ordered by its stream. */
+#define DSEM_NOALLOC_SCRATCH_BYTES (64ull << 20)
+int  dsem_rhs_device_noalloc(dsem_ctx *ctx, const double *dJQ, double *dQt);
+
 int  dsem_state_pointers(dsem_ctx *ctx, double **dJQ, double **dG);
 int  dsem_stream(dsem_ctx *ctx, void **hip_stream);
 int  dsem_synchronize(dsem_ctx *ctx);
