@@ -80,6 +80,9 @@ def load() -> C.CDLL:
         "dsem_last_error": ([vp], C.c_char_p),
         "dsem_sizes": ([vp, _ip, _ip, _ip, C.POINTER(C.c_int64)], C.c_int),
         "dsem_gauss_points": ([vp, _dp, _dp], C.c_int),
+        "dsem_points": ([vp, _dp, _dp], C.c_int),
+        "dsem_operators": ([vp, _dp, _dp], C.c_int),
+        "dsem_get_metrics": ([vp, _dp, _dp, _dp, _dp], C.c_int),
         "dsem_set_state": ([vp, _dp, C.c_int], C.c_int),
         "dsem_get_state": ([vp, _dp, C.c_int], C.c_int),
         "dsem_set_viscosity": ([vp, _dp], C.c_int),

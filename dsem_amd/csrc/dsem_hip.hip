@@ -1982,6 +1982,8 @@ struct dsem_ctx {
     hipStream_t stream = nullptr;
     dsem::Ops ops;
     std::vector<double> h_xg, h_jacob;   // Gauss points and 1/J on the host
+    std::vector<double> h_corners;       // [K][8][3], for host-side geometry queries
+    std::vector<int32_t> h_melem, h_mface;
     // device buffers
     double *JQ = nullptr, *G = nullptr, *Qtp = nullptr, *Qt = nullptr;
     double *T = nullptr, *FV = nullptr, *QA = nullptr, *FI = nullptr, *FT = nullptr;
@@ -2585,6 +2587,9 @@ int create_impl(const dsem_mesh_desc *mesh, const dsem_params *prm, int device, 
     std::vector<double> nhat, norm;
     dsem::mortar_geometry(c->ops, c->M, mesh->corners, mesh->mortar_elem, mesh->mortar_face, nhat, norm);
     c->h_xg = g.xg;
+    c->h_corners.assign(mesh->corners, mesh->corners + (size_t)c->K * 24);
+    c->h_melem.assign(mesh->mortar_elem, mesh->mortar_elem + 2 * (size_t)c->M);
+    c->h_mface.assign(mesh->mortar_face, mesh->mortar_face + 2 * (size_t)c->M);
     c->h_jacob = g.jacob;
     int mode = prm->metrics;
     if (mode == DSEM_METRICS_AUTO) mode = g.affine ? DSEM_METRICS_PER_ELEMENT : DSEM_METRICS_PER_POINT;
@@ -2708,6 +2713,56 @@ int dsem_gauss_points(dsem_ctx *c, double *xg, double *jacob)
     if (!c || !c->stream) return DSEM_ERR_ARG;
     if (xg) std::memcpy(xg, c->h_xg.data(), sizeof(double) * c->h_xg.size());
     if (jacob) std::memcpy(jacob, c->h_jacob.data(), sizeof(double) * c->h_jacob.size());
+    return DSEM_OK;
+}
+
+int dsem_points(const dsem_ctx *c, double *cx, double *cxg)
+{
+    if (!c || !c->stream) return DSEM_ERR_ARG;
+    if (cx) std::memcpy(cx, c->ops.cx, sizeof(double) * c->nc);
+    if (cxg) std::memcpy(cxg, c->ops.cxg, sizeof(double) * c->ncg);
+    return DSEM_OK;
+}
+
+int dsem_operators(const dsem_ctx *c, double *dmx, double *bx)
+{
+    if (!c || !c->stream) return DSEM_ERR_ARG;
+    if (dmx) std::memcpy(dmx, c->ops.D, sizeof(double) * c->ncg * c->nc);
+    if (bx) std::memcpy(bx, c->ops.B, sizeof(double) * c->nc * c->ncg);
+    return DSEM_OK;
+}
+
+int dsem_get_metrics(dsem_ctx *c, double *gmetg, double *gmet, double *nhat, double *norm)
+{
+    if (!c || !c->stream) return DSEM_ERR_ARG;
+    // recomputed on the host from the corners: the same routine and values
+    // as the per-point arrays on the device
+    if (gmetg || gmet) {
+        dsem::Geometry g;
+        dsem::element_geometry(c->ops, c->K, c->h_corners.data(), g);
+        const size_t np = c->np, ns = c->ns;
+        for (size_t e = 0; e < (size_t)c->K; ++e) {
+            if (gmetg)
+                for (size_t pt = 0; pt < np; ++pt)
+                    for (int i = 0; i < 9; ++i) gmetg[(e * np + pt) * 9 + i] = g.gmetg[(e * 9 + i) * np + pt];
+            if (gmet)
+                for (int d = 0; d < 3; ++d)
+                    for (size_t s = 0; s < ns; ++s)
+                        for (int i = 0; i < 3; ++i)
+                            gmet[((e * 3 + d) * ns + s) * 3 + i] = g.gmet[d][(e * 3 + i) * ns + s];
+        }
+    }
+    if (nhat || norm) {
+        std::vector<double> nh, nr;
+        dsem::mortar_geometry(c->ops, c->M, c->h_corners.data(), c->h_melem.data(), c->h_mface.data(), nh, nr);
+        const size_t nf = c->nf;
+        for (size_t mm = 0; mm < (size_t)c->M; ++mm)
+            for (size_t q = 0; q < nf; ++q) {
+                if (nhat)
+                    for (int i = 0; i < 3; ++i) nhat[(mm * nf + q) * 3 + i] = nh[(mm * 3 + i) * nf + q];
+                if (norm) norm[mm * nf + q] = nr[mm * nf + q];
+            }
+    }
     return DSEM_OK;
 }
 

@@ -105,6 +105,53 @@ class DSEM:
         tecplot.write_plt(path, time, xg, self.get_state(), jac, self.p, self.params["gamma"], self.params["mach"],
                           scale, ev, nut)
 
+    # ------------------------------------------------------------- restart files
+    def write_restart(self, path: str, time: float = 0.0) -> None:
+        """Write the current state as the reference's restart file
+        (write_restart, src/Parabolic3D/stac3m/restart.f90:19-312; layout in
+        dsem_amd.restart): J*Q, the lagged staggered states when the context
+        keeps them (shock = 1 or keep_lagged_states), geometry, operators,
+        topology and ``time``."""
+        from . import restart as R
+        M_, nf = self.mesh.nmortar, self.ncg * self.ncg
+        cx, cxg = np.zeros(self.nc), np.zeros(self.ncg)
+        self._check(self._L.dsem_points(self._h, _dp(cx), _dp(cxg)))
+        dmx, bx = np.zeros((self.ncg, self.nc)), np.zeros((self.nc, self.ncg))
+        self._check(self._L.dsem_operators(self._h, _dp(dmx), _dp(bx)))
+        gmetg, gmet = np.zeros((self.K, self.np, 9)), np.zeros((self.K, 3, self.ns, 3))
+        nhat, norm = np.zeros((M_, nf, 3)), np.zeros((M_, nf))
+        self._check(self._L.dsem_get_metrics(self._h, _dp(gmetg), _dp(gmet), _dp(nhat), _dp(norm)))
+        xg, jac = self.gauss_points()
+        try:
+            stag = self.get_lagged_states()
+        except L.DsemError:
+            stag = None
+        R.write_restart(path, self.mesh, self.get_state(), cx=cx, cxg=cxg, dmx=dmx, bx=bx, xg=xg, jacob=jac,
+                        gmetg=gmetg, gmet=gmet, nhat=nhat, norm=norm, time=time, stag=stag)
+
+    def load_restart(self, rst) -> float:
+        """Take the state of a restart file (dsem_amd.restart.read_restart's
+        Restart, same mesh) and, when the context keeps them and the file has
+        them, the lagged staggered states; returns the file's time."""
+        if rst.JQ.shape != (self.K, 5, self.np):
+            raise ValueError("the restart file holds a different mesh")
+        self.set_state(rst.JQ)
+        if np.any(rst.stag != 0.0):
+            try:
+                self.set_lagged_states(rst.stag)
+            except L.DsemError:
+                pass
+        return rst.time
+
+    @classmethod
+    def from_restart(cls, path: str, **kw):
+        """A context on the mesh of a restart file, holding its state; returns
+        (solver, time)."""
+        from . import restart as R
+        rst = R.read_restart(path)
+        s = cls(rst.mesh, **kw)
+        return s, s.load_restart(rst)
+
     def gauss_points(self):
         xg = np.zeros((self.K, self.np, 3))
         jac = np.zeros((self.K, self.np))

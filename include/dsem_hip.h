@@ -119,6 +119,24 @@ int  dsem_sizes(const dsem_ctx *ctx, int32_t *K, int32_t *p, int32_t *nmortar, i
  * (set_domain_geometry.f90:86-97); either pointer may be NULL. */
 int  dsem_gauss_points(dsem_ctx *ctx, double *xg, double *jacob);
 
+/* Chebyshev-Lobatto cx [nc] and Chebyshev-Gauss cxg [ncg] points on [0,1]
+ * (set_points, src/Grid_Routines/stac3m_geomLib.f90:7-47); either may be NULL. */
+int  dsem_points(const dsem_ctx *ctx, double *cx, double *cxg);
+
+/* The Gauss derivative and Gauss->Lobatto interpolation operators of
+ * dermat / intrpmat (src/Spectral/stac3m/matrices.f90:144-207):
+ * dmx [ncg][nc] with dmx[k][j] = dmx(k+1,j+1), bx [nc][ncg] with
+ * bx[j][k] = bx(j+1,k+1); either may be NULL. */
+int  dsem_operators(const dsem_ctx *ctx, double *dmx, double *bx);
+
+/* Per-point metrics of the context's mesh as set_geometry leaves them in
+ * d%gmetg, d%gmet and the mortars (src/Grid_Routines/set_domain_geometry.f90:8-348),
+ * whatever the DSEM_METRICS mode; any pointer may be NULL.
+ *   gmetg [K][ncg^3][9]: entry 3i+j = gmetg(i+1,j+1) at the Gauss points;
+ *   gmet  [K][3][nc*ncg^2][3]: gmet(d+1,c+1) on staggered grid d;
+ *   nhat  [nmortar][ncg^2][3], norm [nmortar][ncg^2] (mrtr%n_hat, mrtr%norm). */
+int  dsem_get_metrics(dsem_ctx *ctx, double *gmetg, double *gmet, double *nhat, double *norm);
+
 int  dsem_set_state(dsem_ctx *ctx, const double *JQ, int layout);
 int  dsem_get_state(dsem_ctx *ctx, double *JQ, int layout);
 

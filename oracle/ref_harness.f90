@@ -197,4 +197,83 @@ contains
       ref_polyn = polyn(k, x, n, z)
    end function ref_polyn
 
+   ! Byte sizes of the restart records write_restart writes
+   ! (src/Parabolic3D/stac3m/restart.f90:59-207), in its order, from the
+   ! reference's compiled declarations of TYPE(domain), TYPE(mortar),
+   ! Order_Matrices and Material_Properties (src/paraFMDF/initializationsc2.f90).
+   ! The harness only measures the declared components; it writes nothing.
+   ! Returns the number of sizes stored in out (at most nmax_out).
+   integer(c_int) function ref_restart_record_sizes(out, nmax_out) bind(C, name='ref_restart_record_sizes')
+      use Material_Properties
+      integer(c_int), value :: nmax_out
+      integer(c_int) :: out(nmax_out)
+      type(mortar), save :: mt
+      integer :: n, ngrid, nmort
+      double precision :: time
+      n = 0
+      call put(storage_size(ngrid) / 8 + storage_size(nmort) / 8)
+      call put(3 * storage_size(num_ordersx) / 8)
+      call put(storage_size(order_mapx) / 8 * (size(order_mapx) + size(order_mapy) + size(order_mapz)))
+      call put(storage_size(bx) / 8 * size(bx))
+      call put(storage_size(by) / 8 * size(by))
+      call put(storage_size(bz) / 8 * size(bz))
+      call put(storage_size(dmx) / 8 * size(dmx))
+      call put(storage_size(dmy) / 8 * size(dmy))
+      call put(storage_size(dmz) / 8 * size(dmz))
+      call put(storage_size(num_mrtr_matrices) / 8)
+      call put(storage_size(mrtr_mat_map) / 8 * size(mrtr_mat_map))
+      call put(storage_size(prolong) / 8 * size(prolong))
+      call put(storage_size(restrict) / 8 * size(restrict))
+      if (.not. allocated(dd)) then
+         allocate(dd)
+      end if
+      call put(storage_size(dd%type) / 8)
+      call put(storage_size(dd%node) / 8 * size(dd%node))
+      call put(storage_size(dd%orientation) / 8 * size(dd%orientation))
+      call put(storage_size(dd%mortar) / 8 * size(dd%mortar))
+      call put(storage_size(dd%which_surface) / 8 * size(dd%which_surface))
+      call put(storage_size(dd%nc) / 8 * (size(dd%nc) + size(dd%ncg)))
+      call put(storage_size(dd%Q) / 8 * size(dd%Q))
+      call put(storage_size(dd%Qlgg) / 8 * size(dd%Qlgg))
+      call put(storage_size(dd%Qglg) / 8 * size(dd%Qglg))
+      call put(storage_size(dd%Qggl) / 8 * size(dd%Qggl))
+      call put(storage_size(dd%f) / 8 * size(dd%f))
+      call put(storage_size(dd%g) / 8 * size(dd%g))
+      call put(storage_size(dd%h) / 8 * size(dd%h))
+      call put(storage_size(dd%g_Q) / 8 * size(dd%g_Q))
+      call put(storage_size(dd%material_id) / 8)
+      call put(storage_size(dd%cx) / 8 * (size(dd%cx) + size(dd%cxg)))
+      call put(storage_size(dd%xg) / 8 * size(dd%xg))
+      call put(storage_size(dd%jacob) / 8 * size(dd%jacob))
+      call put(storage_size(dd%gmet) / 8 * size(dd%gmet))
+      call put(storage_size(dd%gmetg) / 8 * size(dd%gmetg))
+      call put(storage_size(dd%corner) / 8 * size(dd%corner))
+      call put(storage_size(dd%ibtype) / 8 * size(dd%ibtype))
+      call put(storage_size(dd%bcond) / 8 * size(dd%bcond))
+      call put(storage_size(dd%domain_type) / 8)
+      call put(storage_size(mt%lenmortar) / 8 * size(mt%lenmortar))
+      call put(storage_size(mt%len) / 8 * size(mt%len))
+      call put(storage_size(mt%id) / 8 * size(mt%id))
+      call put(storage_size(mt%iface) / 8 * size(mt%iface))
+      call put(storage_size(mt%orient) / 8)
+      call put(storage_size(mt%sign) / 8)
+      call put(storage_size(mt%nsign) / 8)
+      call put(storage_size(mt%n_hat) / 8 * size(mt%n_hat))
+      call put(storage_size(mt%norm) / 8 * size(mt%norm))
+      call put(storage_size(mt%Q) / 8 * size(mt%Q))
+      call put(storage_size(mt%fv) / 8 * size(mt%fv))
+      call put(storage_size(mt%xg) / 8 * size(mt%xg))
+      call put(storage_size(mt%conforming) / 8 * size(mt%conforming))
+      call put(storage_size(num_prop) / 8)
+      call put(storage_size(material_property) / 8 * size(material_property, 1))
+      call put(storage_size(time) / 8)
+      ref_restart_record_sizes = n
+   contains
+      subroutine put(v)
+         integer, intent(in) :: v
+         n = n + 1
+         if (n <= nmax_out) out(n) = v
+      end subroutine put
+   end function ref_restart_record_sizes
+
 end module dsem_ref_harness
