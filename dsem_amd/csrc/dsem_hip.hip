@@ -544,15 +544,36 @@ __device__ __forceinline__ LaneFlux volume_grid(const DevMesh &m, const DevPhys 
     // Matrix-core form: one pass, every lane of an active wave takes part in
     // the contractions (the tiles need all 64 lanes); only lanes that own a
     // point store.  Vector form: work-items stride over the points.
-    const int w0 = MF ? 0 : tid;
-    const int wend = MF ? (ml.wave_active ? 1 : 0) : S::NS;
-    const int wstep = MF ? 1 : VBLOCK;
+    // FP32 mode (PM 1): the 20 contractions of a point (5 interpolated state
+    // fields, 15 prolonged gradients) run on v_mfma_f32_4x4x1f32.  Each 4x4
+    // block is one line and four consecutive Lobatto rows j; lane i of a
+    // block supplies field (4 ch + i) at Gauss point k of the line (A) and
+    // the operator entry B[j_i][k] of its own row (B), and receives fields
+    // 4 ch .. 4 ch + 3 at its own point (D column i, rows in the four
+    // result registers).  Five chains of ncg steps give all 20 fields at the
+    // lane's point; passes cover the ncg^2 lines x ceil(nc/4) row groups.
+    constexpr bool MF32 = (PM == 1);
+    constexpr int JG = (S::NC + 3) / 4;                       // row groups per line
+    constexpr int NB32 = S::NCG * S::NCG * JG;                // blocks per grid
+    constexpr int NPASS32 = (NB32 + VBLOCK / 4 - 1) / (VBLOCK / 4);
+    const int w0 = (MF || MF32) ? 0 : tid;
+    const int wend = MF ? (ml.wave_active ? 1 : 0) : (MF32 ? NPASS32 : S::NS);
+    const int wstep = (MF || MF32) ? 1 : VBLOCK;
 #pragma unroll 1
     for (int w = w0; w < wend; w += wstep) {
         int a, b, j;
         bool valid = true;
         int off[KS];
-        if constexpr (MF) {
+        int line32 = 0;
+        if constexpr (MF32) {
+            const int blk = w * (VBLOCK / 4) + tid / 4;
+            const int line = blk / JG;
+            j = 4 * (blk % JG) + tid % 4;
+            valid = line < S::NF && j < S::NC;
+            line32 = line < S::NF ? line : 0;                 // past the last line: read line 0, keep nothing
+            a = line32 % S::NCG;
+            b = line32 / S::NCG;
+        } else if constexpr (MF) {
             j = ml.j;
             a = ml.line % S::NCG;
             b = ml.line / S::NCG;
@@ -589,7 +610,29 @@ __device__ __forceinline__ LaneFlux volume_grid(const DevMesh &m, const DevPhys 
         }
         double Qv[5];
         CT Bj[MF ? 1 : S::NCG];
-        if constexpr (MF) {
+        float fld32[MF32 ? 20 : 1];
+        if constexpr (MF32) {
+            typedef float f4 __attribute__((ext_vector_type(4)));
+            float opk[S::NCG];
+            const int jr = j < S::NC ? j : S::NC - 1;
+#pragma unroll
+            for (int k = 0; k < S::NCG; ++k) opk[k] = j < S::NC ? sB[jr * S::NCG + k] : 0.0f;
+            int goff[S::NCG];
+#pragma unroll
+            for (int k = 0; k < S::NCG; ++k) goff[k] = gidx<P>(D, k, a, b);
+#pragma unroll
+            for (int ch = 0; ch < 5; ++ch) {
+                const int F = 4 * ch + tid % 4;
+                const CT *src = F < 5 ? sQ + F * S::NP : sG + (F - 5) * S::NP;
+                f4 t = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+                for (int k = 0; k < S::NCG; ++k) t = __builtin_amdgcn_mfma_f32_4x4x1f32(src[goff[k]], opk[k], t, 0, 0, 0);
+#pragma unroll
+                for (int r = 0; r < 4; ++r) fld32[4 * ch + r] = t[r];
+            }
+#pragma unroll
+            for (int v = 0; v < 5; ++v) Qv[v] = face ? vqa[v] : sBar[v] + (double)fld32[v];
+        } else if constexpr (MF) {
 #pragma unroll
             for (int v = 0; v < 5; ++v) {
                 const double t = mf_line<P>(sQ + v * S::NP, ml, off);
@@ -627,7 +670,7 @@ __device__ __forceinline__ LaneFlux volume_grid(const DevMesh &m, const DevPhys 
         // one direction at a time in the vector form, whose registers would spill.
         double Sxx = 0.0, Syy = 0.0, Szz = 0.0, Sxy = 0.0, Sxz = 0.0, Syz = 0.0, div = 0.0;
         double Tx = 0.0, Ty = 0.0, Tz = 0.0;
-        constexpr int CUNROLL = MF ? 3 : 1;
+        constexpr int CUNROLL = (MF || MF32) ? 3 : 1;   // MF32: the fields are in registers already
 #pragma unroll CUNROLL
         for (int c = 0; c < 3; ++c) {
             double Gc[5];
@@ -636,6 +679,9 @@ __device__ __forceinline__ LaneFlux volume_grid(const DevMesh &m, const DevPhys 
                 const CT *gl = sG + (c * 5 + v) * S::NP;
                 if constexpr (MF) {
                     Gc[v] = mf_line<P>(gl, ml, off);
+                } else if constexpr (MF32) {
+                    (void)gl;
+                    Gc[v] = (double)fld32[5 + c * 5 + v];
                 } else {
                     CA u = 0;
 #pragma unroll
@@ -906,7 +952,7 @@ __device__ __forceinline__ void volume_inputs(const double *__restrict__ JQ, con
 // Pointwise physics, the gradient metric combination inputs, the RK state
 // and all face kernels stay FP64.
 template <int P, int PM>
-__global__ __launch_bounds__((vblock<P, PM>())) void k_volume(const double *__restrict__ JQ, DevMesh m, DevPhys ph,
+__global__ __launch_bounds__((vblock<P, PM>()), 4) void k_volume(const double *__restrict__ JQ, DevMesh m, DevPhys ph,
                                                     const double *__restrict__ QA, const double *__restrict__ muh,
                                                     const double *__restrict__ nut, double *__restrict__ FV,
                                                     double *__restrict__ Qtp, double *__restrict__ Slag)

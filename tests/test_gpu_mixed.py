@@ -2,7 +2,7 @@
 
 FP32 contractions on element-shifted fields: one RHS of the throughput input
 (TGV with a seeded 1e-3 relative perturbation) within 1e-6 relative L2 per
-conserved field, and the state after 200 steps from the clean TGV field
+conserved field, and the state after 1000 steps from the clean TGV field
 within 1e-6 relative L2 per field.  BF16x3 is checked against looser bounds
 (its split products keep ~16 significant bits; rho w starts at zero in the
 TGV, so its relative state error is the largest).  The clean TGV RHS itself is
@@ -40,11 +40,28 @@ def test_mixed_rhs_and_state_against_fp64(mode):
         s64.set_state(clean)
         dt = s64.dtcalc(0.5)
         smx.set_state(clean)
-        s64.steps(200, dt)
-        smx.steps(200, dt)
+        s64.steps(1000, dt)
+        smx.steps(1000, dt)
         e = field_errors(smx.get_state(), s64.get_state())
         print("state", {k: round(v["rel_l2"], 12) for k, v in e.items()}, flush=True)
         assert all(v["rel_l2"] < state_bound for v in e.values())
+
+
+@pytest.mark.parametrize("p", [1, 2, 3, 4, 5, 6, 7])
+def test_fp32_rhs_every_order(p):
+    """FP32 contractions (matrix cores) at every order on a distorted mesh,
+    per-point metrics: one RHS within 1e-6 of the FP64 path per field."""
+    from helpers import distorted_box
+    print(f"\n[case] fp32 rhs p={p}", flush=True)
+    mesh = distorted_box(3, p, amp=0.05, seed=p)
+    with DSEM(mesh, re=800.0) as s64, DSEM(mesh, re=800.0, precision="fp32") as s32:
+        xg, jac = s64.gauss_points()
+        JQ = cases.tgv_state(xg, jac, "isothermal", perturb=1e-2, seed=p)
+        s64.set_state(JQ)
+        s32.set_state(JQ)
+        e = field_errors(s32.rhs(), s64.rhs())
+    print({k: v["rel_l2"] for k, v in e.items()}, flush=True)
+    assert all(v["rel_l2"] < 1e-6 for v in e.values())
 
 
 def test_mixed_precision_is_rejected_for_adjoint():
