@@ -114,16 +114,26 @@ def cpu_baseline(mesh, JQ, args) -> dict:
 
 def mixed_modes(mesh, JQ, s64, dt, args, ms64) -> dict:
     """Throughput of the FP32 / BF16x3 contraction modes on the same input,
-    and their per-field relative L2 error of one RHS against the FP64 path."""
+    their per-field relative L2 error of one RHS against the FP64 path, and
+    of the state after ``args.mixed_steps`` steps from the clean TGV field."""
     from dsem_amd.mixed_study import field_errors
     from dsem_amd.solver import DSEM
     s64.set_state(JQ)
     r64 = s64.rhs()
+    xg, jac = s64.gauss_points()
+    clean = cases.tgv_state(xg, jac, "reference")
+    s64.set_state(clean)
+    s64.steps(args.mixed_steps, dt, check=False)
+    q64 = s64.get_state()
     res = {}
     for mode in ("fp32", "bf16x3"):
         with DSEM(mesh, re=1600.0, pr=0.71, mach=1.0, precision=mode, metrics=args.metrics) as sm:
             sm.set_state(JQ)
             err = field_errors(sm.rhs(), r64)
+            sm.set_state(clean)
+            sm.steps(args.mixed_steps, dt, check=False)
+            errs = field_errors(sm.get_state(), q64)
+            sm.set_state(JQ)
             sm.steps(2, dt, check=False)
             sm.synchronize()
             t0 = time.perf_counter()
@@ -135,7 +145,11 @@ def mixed_modes(mesh, JQ, s64, dt, args, ms64) -> dict:
                      "ms_per_step": round(ms, 4), "speedup_vs_fp64": round(ms64 / ms, 3),
                      "rel_l2_max_over_fields": max(v["rel_l2"] for v in err.values()),
                      "rel_l2": {k: v["rel_l2"] for k, v in err.items()},
-                     "rel_linf": {k: v["rel_linf"] for k, v in err.items()}}
+                     "rel_linf": {k: v["rel_linf"] for k, v in err.items()},
+                     f"state_after_{args.mixed_steps}_steps": {
+                         "rel_l2": {k: v["rel_l2"] for k, v in errs.items()},
+                         "rel_linf": {k: v["rel_linf"] for k, v in errs.items()},
+                         "input": "clean TGV field (reference IC), same dt as the timed run"}}
     return res
 
 
@@ -152,7 +166,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-roofline", action="store_true")
     ap.add_argument("--no-mixed", action="store_true", help="skip the FP32 / BF16x3 contraction modes")
-    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint cost/gradient check (8^3, p=6)")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint cost/gradient checks")
+    ap.add_argument("--mixed-steps", type=int, default=1000, help="steps of the mixed-precision state comparison")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="run one device-to-device copy kernel of this many bytes first (counter calibration)")
     args = ap.parse_args()
@@ -262,6 +277,14 @@ def main():
                           "config": "TGV 8^3 hex8, p=6, Re=1600, 50 steps, checkpoint every 10, "
                                     "J = int 1/2 rho|u|^2 dV; dJ/du0 on 4 seeded directions and dJ/dnu vs "
                                     "central differences"}
+        r = adjoint_study.run(n=args.elements, p=args.p, steps=10, every=5, ndir=2)
+        out["adjoint_production_size"] = {
+            "adjoint_over_forward": round(r["adjoint_over_forward"], 3), "forward_ms": round(r["forward_ms"], 2),
+            "backward_ms": round(r["backward_ms"], 2), "grad_fd_max_rel_err": r["velocity_directions"]["max_rel_err"],
+            "nu_fd_rel_err": r["nu"]["rel_err"],
+            "config": f"TGV {args.elements}^3 hex8, p={args.p}, Re=1600, 10 steps, checkpoint every 5 "
+                      "(uniform binomial schedule), J = int 1/2 rho|u|^2 dV; dJ/du0 on 2 seeded directions and "
+                      "dJ/dnu vs central differences"}
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(mesh, JQ, args)
     print(json.dumps(out), flush=True)

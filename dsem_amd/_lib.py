@@ -105,6 +105,10 @@ def load() -> C.CDLL:
         "dsem_tape_reserve": ([vp, C.c_int], C.c_int),
         "dsem_rk_step_record": ([vp, C.c_double, C.c_int], C.c_int),
         "dsem_rk_step_vjp_tape": ([vp, C.c_double, C.c_int, vp, _dp], C.c_int),
+        "dsem_rk_step_vjp_tape_lagged": ([vp, C.c_double, C.c_int, vp, vp, _dp], C.c_int),
+        "dsem_ev_vjp": ([vp, _dp, _dp, _dp], C.c_int),
+        "dsem_get_lagged_states_device": ([vp, vp, _ip], C.c_int),
+        "dsem_set_lagged_states_device": ([vp, vp], C.c_int),
         "dsem_set_state_device": ([vp, vp], C.c_int),
         "dsem_get_state_device": ([vp, vp], C.c_int),
         "dsem_quadrature_weights": ([vp, _dp], C.c_int),

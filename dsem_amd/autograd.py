@@ -73,37 +73,38 @@ class RKSolve(torch.autograd.Function):
             snaps = -(-nsteps // every) - 1
         plan = Plan(nsteps, max(0, int(snaps)), tape)
         keep = set(plan.forward_snapshots())
+        ev = bool(solver.params["shock"])
         torch.cuda.synchronize()
-        solver.set_state_device(JQ0.data_ptr())
-        held = {0: JQ0.detach().clone()}
+        solver.set_state_device(JQ0.data_ptr())      # a fresh start: the first step's mu_h is zero
+        held = {0: _snapshot(solver, JQ0.detach().clone(), ev, initial=True)}
         for s in range(1, nsteps + 1):
             solver.steps(1, dt, check=False)
             if s in keep:
-                t = torch.empty_like(JQ0)
-                solver.get_state_device(t.data_ptr())
-                held[s] = t
+                held[s] = _snapshot(solver, None, ev)
         out = torch.empty_like(JQ0)
         solver.get_state_device(out.data_ptr())
-        ctx.solver, ctx.dt, ctx.plan, ctx.held = solver, dt, plan, held
+        ctx.solver, ctx.dt, ctx.plan, ctx.held, ctx.ev = solver, dt, plan, held, ev
         return out
 
     @staticmethod
     def backward(ctx, gout: torch.Tensor):
-        solver, dt, plan, held = ctx.solver, ctx.dt, ctx.plan, ctx.held
+        solver, dt, plan, held, ev = ctx.solver, ctx.dt, ctx.plan, ctx.held, ctx.ev
         qbar = gout.detach().to(torch.float64).contiguous().clone()
+        # with entropy viscosity the adjoint state also carries the lagged
+        # staggered states' adjoint (zero after the last step: J reads the
+        # state only)
+        slagbar = torch.zeros((solver.K, 3, 5, solver.ns), dtype=torch.float64, device=qbar.device) if ev else None
         torch.cuda.synchronize()
         nubar = 0.0
         solver.tape_reserve(plan.tape)
         for act in plan.actions():
             kind = act[0]
             if kind == "restore":
-                solver.set_state_device(held[act[1]].data_ptr())
+                _restore(solver, held[act[1]], ev)
             elif kind == "advance":
                 solver.steps(act[1], dt, check=False)
             elif kind == "snapshot":
-                t = torch.empty_like(qbar)
-                solver.get_state_device(t.data_ptr())
-                held[act[1]] = t
+                held[act[1]] = _snapshot(solver, None, ev)
             elif kind == "free":
                 del held[act[1]]
             else:   # reverse: record k steps from the current state, then their adjoints backwards
@@ -111,9 +112,37 @@ class RKSolve(torch.autograd.Function):
                 for i in range(k):
                     solver.rk_step_record(dt, i)
                 for i in reversed(range(k)):
-                    nubar += solver.rk_step_vjp_tape(dt, i, qbar.data_ptr())
+                    if ev:
+                        nubar += solver.rk_step_vjp_tape_lagged(dt, i, qbar.data_ptr(), slagbar.data_ptr())
+                    else:
+                        nubar += solver.rk_step_vjp_tape(dt, i, qbar.data_ptr())
         ctx.held = None
         return qbar, torch.tensor(nubar, dtype=torch.float64, device=qbar.device), None, None, None, None, None, None
+
+
+def _snapshot(solver: DSEM, state, ev: bool, initial: bool = False):
+    """The solver's current state (or the given tensor) and, with entropy
+    viscosity, its lagged staggered states (None on a fresh start)."""
+    if state is None:
+        state = torch.empty((solver.K, 5, solver.np), dtype=torch.float64, device="cuda")
+        solver.get_state_device(state.data_ptr())
+    if not ev:
+        return state
+    lag = None
+    if not initial:
+        lag = torch.empty((solver.K, 3, 5, solver.ns), dtype=torch.float64, device=state.device)
+        if not solver.get_lagged_states_device(lag.data_ptr()):
+            lag = None
+    return (state, lag)
+
+
+def _restore(solver: DSEM, snap, ev: bool) -> None:
+    if not ev:
+        solver.set_state_device(snap.data_ptr())
+        return
+    state, lag = snap
+    solver.set_state_device(state.data_ptr())
+    solver.set_lagged_states_device(None if lag is None else lag.data_ptr())
 
 
 def kinetic_energy_integral(JQ: torch.Tensor, bmat: torch.Tensor) -> torch.Tensor:

@@ -2050,6 +2050,8 @@ struct dsem_ctx {
     double *aQVb = nullptr, *aGVb = nullptr, *aFTbe = nullptr, *aFVb = nullptr, *aQAbe = nullptr;
     double *aTb = nullptr, *aFIb = nullptr, *aJQvb = nullptr, *aNuPart = nullptr, *aNu = nullptr;
     double *aGb = nullptr, *aLam = nullptr, *aOut = nullptr;
+    double *aMuB = nullptr, *aWB = nullptr, *aSlagB = nullptr;   // entropy-viscosity adjoint (shock = 1)
+    std::vector<char> tape_lag;                // per tape slot: the recorded step had lagged states
     // step tape: per slot and stage the stage input state and the forward
     // intermediates the adjoint reads (face traces T, mortar averages QA,
     // viscous face traces FV)
@@ -2377,6 +2379,11 @@ int adjoint_alloc(dsem_ctx *c)
     TRY(dev_alloc(c, &c->aGb, n));
     TRY(dev_alloc(c, &c->aLam, n));
     TRY(dev_alloc(c, &c->aOut, n));
+    if (c->params.shock) {
+        TRY(dev_alloc(c, &c->aMuB, K * 3 * c->ns));
+        TRY(dev_alloc(c, &c->aWB, K * 18 * c->np));
+        TRY(dev_alloc(c, &c->aSlagB, K * 3 * NEQ * c->ns));
+    }
     return DSEM_OK;
 }
 
@@ -2390,9 +2397,12 @@ struct FwdTape {
 // Plain VJP (Qb == nullptr): Out = (dR/dJQ)^T Gb.  RK stage reverse
 // (Qb != nullptr): Gb <- Gb + cdt Qb, Qb += (dR/dJQ)^T Gb, Gb <- a Gb.
 // nu_slot receives <Gb, dR/dnu> (device, one double).
+// With entropy viscosity: muh is the step's mu_h (staggered), MuB
+// accumulates its adjoint, and SlagB (the adjoint of the lagged states this
+// evaluation leaves, last stage only) joins the staggered-state adjoints.
 template <int P>
 int run_rhs_vjp(dsem_ctx *c, double *JQ, FwdTape f, bool fwd, double *Gb, double *Qb, double cdt, double a,
-                double *Out, double *nu_slot)
+                double *Out, double *nu_slot, const double *muh, double *MuB, const double *SlagB)
 {
     using S = Sz<P>;
     const DevMesh m = dev_mesh(c);
@@ -2409,7 +2419,7 @@ int run_rhs_vjp(dsem_ctx *c, double *JQ, FwdTape f, bool fwd, double *Gb, double
     return launch(c, KID_ADJ, [&] {
         k_adj_update<P><<<K, BLOCK, 0, s>>>(Gb, Qb, cdt, c->aFTbe);
         k_adj_face_visc<P><<<face_blocks, BLOCK, 0, s>>>(c->aFTbe, f.FV, m, ph, c->aFIb, c->aFVb, nu_f);
-        k_adj_volume<P><<<K, ABLOCK, 0, s>>>(JQ, m, ph, f.QA, nullptr, Gb, c->aFVb, c->aQVb, c->aGVb, nu_v);
+        k_adj_volume<P><<<K, ABLOCK, 0, s>>>(JQ, m, ph, f.QA, muh, Gb, c->aFVb, c->aQVb, c->aGVb, nu_v, MuB, SlagB);
         k_adj_gather<P><<<K, GBLOCK, 0, s>>>(m, c->aQVb, c->aGVb, c->aJQvb, c->aQAbe);
         k_adj_face_inv<P><<<face_blocks, BLOCK, 0, s>>>(f.T, m, ph, c->aFIb, c->aQAbe, c->aTb);
         k_adj_traces<P><<<K, BLOCK, 0, s>>>(c->aTb, c->aJQvb, m, Out, Qb, Gb, a);
@@ -2418,9 +2428,42 @@ int run_rhs_vjp(dsem_ctx *c, double *JQ, FwdTape f, bool fwd, double *Gb, double
 }
 
 int rhs_vjp_dispatch(dsem_ctx *c, double *JQ, FwdTape f, bool fwd, double *Gb, double *Qb, double cdt, double a,
-                     double *Out, double *nu_slot)
+                     double *Out, double *nu_slot, const double *muh = nullptr, double *MuB = nullptr,
+                     const double *SlagB = nullptr)
 {
-    DISPATCH_P(c, run_rhs_vjp, JQ, f, fwd, Gb, Qb, cdt, a, Out, nu_slot);
+    DISPATCH_P(c, run_rhs_vjp, JQ, f, fwd, Gb, Qb, cdt, a, Out, nu_slot, muh, MuB, SlagB);
+}
+
+EvArgs ev_args(const dsem_ctx *c)
+{
+    EvArgs ev;
+    ev.cmu = c->params.cmu;
+    ev.mumax = c->params.mumax;
+    ev.cmax = c->params.cmax;
+    ev.ducros = (c->params.ev_flags & DSEM_EV_DUCROS) != 0;
+    ev.smoothing = (c->params.ev_flags & DSEM_EV_SMOOTHING) != 0;
+    ev.cmax_cap = (c->params.ev_flags & DSEM_EV_CMAX_CAP) != 0;
+    return ev;
+}
+
+// Adjoint of the entropy-viscosity update (k_ev) at state JQ and lagged
+// states Slag: JQb += (d mu_h / d JQ)^T MuB, SlagB = (d mu_h / d Slag)^T MuB.
+template <int P>
+int run_ev_vjp(dsem_ctx *c, const double *JQ, const double *Slag, const double *MuB, double *JQb, double *SlagB)
+{
+    const DevMesh m = dev_mesh(c);
+    const DevPhys ph = dev_phys(c);
+    const EvArgs ev = ev_args(c);
+    hipStream_t s = c->stream;
+    return launch(c, KID_ADJ, [&] {
+        k_adj_ev<P><<<c->K, BLOCK, 0, s>>>(JQ, Slag, m, ph, ev, MuB, JQb, c->aWB);
+        k_adj_ev_gather<P><<<c->K, BLOCK, 0, s>>>(Slag, c->aWB, ph, SlagB);
+    });
+}
+
+int ev_vjp_dispatch(dsem_ctx *c, const double *JQ, const double *Slag, const double *MuB, double *JQb, double *SlagB)
+{
+    DISPATCH_P(c, run_ev_vjp, JQ, Slag, MuB, JQb, SlagB);
 }
 
 // Tape layout per slot and stage: state [n], T [nT], QA [nQA], FV [nT].
@@ -2431,11 +2474,30 @@ size_t tape_stage_size(const dsem_ctx *c)
     return n + 2 * nT + nQA;
 }
 
+// Entropy viscosity: after a slot's five stages, the step's mu_h
+// [K][3][ns] and the lagged states it was computed from [K][3][5][ns].
+size_t tape_ev_size(const dsem_ctx *c)
+{
+    return c->params.shock ? (size_t)c->K * 3 * c->ns * (1 + NEQ) : 0;
+}
+
+size_t tape_slot_size(const dsem_ctx *c)
+{
+    return 5 * tape_stage_size(c) + tape_ev_size(c);
+}
+
+void tape_ev(const dsem_ctx *c, int slot, double **muh, double **slag)
+{
+    double *b = c->tape + (size_t)slot * tape_slot_size(c) + 5 * tape_stage_size(c);
+    *muh = b;
+    *slag = b + (size_t)c->K * 3 * c->ns;
+}
+
 void tape_stage(const dsem_ctx *c, int slot, int k, double **S, FwdTape *f)
 {
     const size_t n = (size_t)c->K * NEQ * c->np, nT = (size_t)c->K * 6 * NEQ * c->nf;
     const size_t nQA = (size_t)c->M * NEQ * c->nf;
-    double *b = c->tape + ((size_t)slot * 5 + k) * tape_stage_size(c);
+    double *b = c->tape + (size_t)slot * tape_slot_size(c) + (size_t)k * tape_stage_size(c);
     *S = b;
     f->T = b + n;
     f->QA = b + n + nT;
@@ -2453,8 +2515,9 @@ int tape_reserve(dsem_ctx *c, int nslots)
         c->tape = nullptr;
         c->tape_slots = 0;
     }
-    TRY(dev_alloc(c, &c->tape, (size_t)nslots * 5 * tape_stage_size(c)));
+    TRY(dev_alloc(c, &c->tape, (size_t)nslots * tape_slot_size(c)));
     c->tape_slots = nslots;
+    c->tape_lag.assign(nslots, 0);
     return DSEM_OK;
 }
 
@@ -2466,6 +2529,19 @@ int record_step(dsem_ctx *c, double dt, int slot)
     const size_t n = (size_t)c->K * NEQ * c->np;
     double *T0 = c->T, *QA0 = c->QA, *FV0 = c->FV;
     int rc = DSEM_OK;
+    if (c->params.shock) {
+        // the step's entropy viscosity (tstep's Entropy call) and the lagged
+        // states it reads, kept for the adjoint
+        double *tm, *ts;
+        tape_ev(c, slot, &tm, &ts);
+        c->tape_lag[slot] = c->have_lag ? 1 : 0;
+        if (c->have_lag)
+            HIP_TRY(c, hipMemcpyAsync(ts, c->Slag, sizeof(double) * (size_t)c->K * 3 * NEQ * c->ns,
+                                      hipMemcpyDeviceToDevice, c->stream));
+        TRY(ev_step(c, false));
+        HIP_TRY(c, hipMemcpyAsync(tm, c->muh, sizeof(double) * (size_t)c->K * 3 * c->ns, hipMemcpyDeviceToDevice,
+                                  c->stream));
+    }
     for (int k = 0; k < 5 && rc == DSEM_OK; ++k) {
         double *S;
         FwdTape f;
@@ -2482,26 +2558,58 @@ int record_step(dsem_ctx *c, double dt, int slot)
         c->T = f.T;
         c->QA = f.QA;
         c->FV = f.FV;
-        rc = rhs_dispatch(c, c->JQ, st, nullptr, false, false, false);
+        // the last stage leaves the next step's lagged states, as dsem_rk_steps does
+        rc = rhs_dispatch(c, c->JQ, st, nullptr, false, false, c->params.shock && k == 4);
         c->T = T0;
         c->QA = QA0;
         c->FV = FV0;
     }
+    if (rc == DSEM_OK && c->params.shock) c->have_lag = true;
+    c->traces_cur = false;
     return rc;
 }
 
 // Adjoint of the step recorded in `slot`: dQbar (device) from the adjoint of
 // the state after the step to that of the state before it; nu_host += the
 // step's <., dR/dnu>.
-int vjp_recorded_step(dsem_ctx *c, double dt, int slot, double *dQbar, double *nu_host)
+// With entropy viscosity, dSlagbar (device, nullable) holds on entry the
+// adjoint of the lagged states the step leaves (its last stage's
+// face-averaged staggered states) and on return that of the lagged states
+// the step's mu_h was computed from; mu_h's own adjoint, accumulated over
+// the five stages, goes back through k_ev's adjoint into dQbar (mu_h is a
+// function of the step's start state) and dSlagbar.
+int vjp_recorded_step(dsem_ctx *c, double dt, int slot, double *dQbar, double *nu_host, double *dSlagbar = nullptr)
 {
     const size_t n = (size_t)c->K * NEQ * c->np;
+    const bool ev = c->params.shock;
+    double *tm = nullptr, *ts = nullptr;
+    if (ev) {
+        tape_ev(c, slot, &tm, &ts);
+        HIP_TRY(c, hipMemsetAsync(c->aMuB, 0, sizeof(double) * (size_t)c->K * 3 * c->ns, c->stream));
+    }
     HIP_TRY(c, hipMemsetAsync(c->aGb, 0, sizeof(double) * n, c->stream));
     for (int k = 4; k >= 0; --k) {
         double *S;
         FwdTape f;
         tape_stage(c, slot, k, &S, &f);
-        TRY(rhs_vjp_dispatch(c, S, f, false, c->aGb, dQbar, kCrk[k] * dt, kArk[k], nullptr, c->aNu + 1 + k));
+        TRY(rhs_vjp_dispatch(c, S, f, false, c->aGb, dQbar, kCrk[k] * dt, kArk[k], nullptr, c->aNu + 1 + k,
+                             ev ? tm : nullptr, ev ? c->aMuB : nullptr, (ev && k == 4) ? dSlagbar : nullptr));
+    }
+    if (ev && dSlagbar) {
+        if (c->tape_lag[slot]) {
+            double *S0;
+            FwdTape f0;
+            tape_stage(c, slot, 0, &S0, &f0);
+            TRY(ev_vjp_dispatch(c, S0, ts, c->aMuB, dQbar, dSlagbar));
+        } else {
+            // a fresh start: mu_h = 0 whatever the state
+            HIP_TRY(c, hipMemsetAsync(dSlagbar, 0, sizeof(double) * (size_t)c->K * 3 * NEQ * c->ns, c->stream));
+        }
+    } else if (ev && c->tape_lag[slot]) {
+        double *S0;
+        FwdTape f0;
+        tape_stage(c, slot, 0, &S0, &f0);
+        TRY(ev_vjp_dispatch(c, S0, ts, c->aMuB, dQbar, c->aSlagB));
     }
     double h[5];
     HIP_TRY(c, hipMemcpyAsync(h, c->aNu + 1, sizeof(h), hipMemcpyDeviceToHost, c->stream));
@@ -3112,12 +3220,14 @@ int dsem_rk_step(dsem_ctx *c, double t, double dt, double *resid_out)
     return dsem_rk_steps(c, t, dt, 1, resid_out);
 }
 
-static int vjp_check(dsem_ctx *c)
+static int vjp_check(dsem_ctx *c, bool allow_shock = false)
 {
     if (!c || !c->stream) return set_err(c, DSEM_ERR_ARG, "vjp arguments");
-    if (c->params.shock || c->les)
+    if ((c->params.shock && !allow_shock) || c->les)
         return set_err(c, DSEM_ERR_UNSUPPORTED,
-                       "the adjoint covers the path without entropy viscosity or Smagorinsky model");
+                       c->les ? "the adjoint does not cover the Smagorinsky model"
+                              : "with entropy viscosity use the tape entry points (dsem_rk_step_record, "
+                                "dsem_rk_step_vjp_tape_lagged) or dsem_ev_vjp");
     if (c->params.precision != DSEM_PREC_FP64)
         return set_err(c, DSEM_ERR_UNSUPPORTED, "the adjoint is built for the FP64 path");
     if (c->rotated)
@@ -3168,14 +3278,14 @@ int dsem_rk_step_vjp_device(dsem_ctx *c, double dt, double *dQbar, double *nubar
 
 int dsem_tape_reserve(dsem_ctx *c, int nslots)
 {
-    TRY(vjp_check(c));
+    TRY(vjp_check(c, true));
     if (nslots < 1) return set_err(c, DSEM_ERR_ARG, "tape_reserve: nslots >= 1");
     return tape_reserve(c, nslots);
 }
 
 int dsem_rk_step_record(dsem_ctx *c, double dt, int slot)
 {
-    TRY(vjp_check(c));
+    TRY(vjp_check(c, true));
     if (slot < 0 || slot >= c->tape_slots) return set_err(c, DSEM_ERR_ARG, "rk_step_record: slot out of range");
     TRY(record_step(c, dt, slot));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
@@ -3187,6 +3297,56 @@ int dsem_rk_step_vjp_tape(dsem_ctx *c, double dt, int slot, double *dQbar, doubl
     TRY(vjp_check(c));
     if (slot < 0 || slot >= c->tape_slots || !dQbar) return set_err(c, DSEM_ERR_ARG, "rk_step_vjp_tape arguments");
     return vjp_recorded_step(c, dt, slot, dQbar, nubar);
+}
+
+int dsem_rk_step_vjp_tape_lagged(dsem_ctx *c, double dt, int slot, double *dQbar, double *dSlagbar, double *nubar)
+{
+    TRY(vjp_check(c, true));
+    if (slot < 0 || slot >= c->tape_slots || !dQbar) return set_err(c, DSEM_ERR_ARG, "rk_step_vjp_tape_lagged arguments");
+    if (c->params.shock && !dSlagbar) return set_err(c, DSEM_ERR_ARG, "rk_step_vjp_tape_lagged: dSlagbar needed with shock = 1");
+    return vjp_recorded_step(c, dt, slot, dQbar, nubar, c->params.shock ? dSlagbar : nullptr);
+}
+
+int dsem_ev_vjp(dsem_ctx *c, const double *muhbar, double *JQbar, double *Slagbar)
+{
+    TRY(vjp_check(c, true));
+    if (!c->params.shock || !c->have_lag) return set_err(c, DSEM_ERR_ARG, "ev_vjp needs shock = 1 and lagged states");
+    if (!muhbar || !JQbar || !Slagbar) return set_err(c, DSEM_ERR_ARG, "ev_vjp pointers");
+    const size_t n = (size_t)c->K * NEQ * c->np, nm = (size_t)c->K * 3 * c->ns, nl = nm * NEQ;
+    HIP_TRY(c, hipMemcpyAsync(c->aMuB, muhbar, sizeof(double) * nm, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, hipMemsetAsync(c->aOut, 0, sizeof(double) * n, c->stream));
+    TRY(ev_vjp_dispatch(c, c->JQ, c->Slag, c->aMuB, c->aOut, c->aSlagB));
+    HIP_TRY(c, hipMemcpyAsync(JQbar, c->aOut, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(Slagbar, c->aSlagB, sizeof(double) * nl, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return DSEM_OK;
+}
+
+int dsem_get_lagged_states_device(dsem_ctx *c, double *dstag, int32_t *valid)
+{
+    if (!c || !c->stream || !c->Slag) return set_err(c, DSEM_ERR_ARG, "lagged states need shock = 1 or dsem_keep_lagged_states");
+    HIP_TRY(c, hipSetDevice(c->device));
+    if (valid) *valid = c->have_lag ? 1 : 0;
+    if (dstag && c->have_lag)
+        HIP_TRY(c, hipMemcpyAsync(dstag, c->Slag, sizeof(double) * (size_t)c->K * 3 * NEQ * c->ns,
+                                  hipMemcpyDeviceToDevice, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return DSEM_OK;
+}
+
+int dsem_set_lagged_states_device(dsem_ctx *c, const double *dstag)
+{
+    if (!c || !c->stream || !c->Slag) return set_err(c, DSEM_ERR_ARG, "lagged states need shock = 1 or dsem_keep_lagged_states");
+    HIP_TRY(c, hipSetDevice(c->device));
+    if (!dstag) {
+        c->have_lag = false;   // as at a fresh start: the next step's mu_h is zero
+        return DSEM_OK;
+    }
+    HIP_TRY(c, hipMemcpyAsync(c->Slag, dstag, sizeof(double) * (size_t)c->K * 3 * NEQ * c->ns,
+                              hipMemcpyDeviceToDevice, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    c->have_lag = true;
+    return DSEM_OK;
 }
 
 int dsem_rk_step_vjp(dsem_ctx *c, double dt, double *Qbar, double *nubar)

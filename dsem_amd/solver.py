@@ -304,6 +304,34 @@ class DSEM:
         self._check(self._L.dsem_rk_step_vjp_tape(self._h, dt, slot, qbar_ptr, C.byref(nb)))
         return nb.value
 
+    # ---------------------------------------------- adjoint with entropy viscosity
+    def ev_vjp(self, muhbar: np.ndarray):
+        """((d mu_h/d JQ)^T muhbar, (d mu_h/d Slag)^T muhbar) at the current
+        state and lagged states (shock = 1); muhbar [K][3][nc ncg^2]."""
+        muhbar = np.ascontiguousarray(muhbar, dtype=np.float64)
+        assert muhbar.size == self.K * 3 * self.ns
+        jb = np.zeros((self.K, 5, self.np))
+        sb = np.zeros((self.K, 3, 5, self.ns))
+        self._check(self._L.dsem_ev_vjp(self._h, _dp(muhbar), _dp(jb), _dp(sb)))
+        return jb, sb
+
+    def rk_step_vjp_tape_lagged(self, dt: float, slot: int, qbar_ptr: int, slagbar_ptr: int) -> float:
+        nb = C.c_double(0.0)
+        self._check(self._L.dsem_rk_step_vjp_tape_lagged(self._h, dt, slot, qbar_ptr, slagbar_ptr, C.byref(nb)))
+        return nb.value
+
+    def get_lagged_states_device(self, ptr: int) -> bool:
+        """Copy the lagged states to device memory at ptr; False (nothing
+        copied) when the context holds none (a fresh start)."""
+        valid = C.c_int32(0)
+        self._check(self._L.dsem_get_lagged_states_device(self._h, ptr, C.byref(valid)))
+        return bool(valid.value)
+
+    def set_lagged_states_device(self, ptr: int | None) -> None:
+        """Lagged states from device memory; None makes the next step a fresh
+        start (mu_h = 0)."""
+        self._check(self._L.dsem_set_lagged_states_device(self._h, ptr))
+
     def set_state_device(self, ptr: int) -> None:
         self._check(self._L.dsem_set_state_device(self._h, ptr))
 

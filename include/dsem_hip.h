@@ -215,6 +215,29 @@ int  dsem_tape_reserve(dsem_ctx *ctx, int nslots);
 int  dsem_rk_step_record(dsem_ctx *ctx, double dt, int slot);
 int  dsem_rk_step_vjp_tape(dsem_ctx *ctx, double dt, int slot, double *dQbar, double *nubar);
 
+/* Adjoint with entropy viscosity (shock = 1).  No counterpart in the
+ * reference.  tstep computes mu_h once per step from the step's start state
+ * and the lagged staggered states the previous step's last stage left
+ * (Entropy, src/EV_routines/Main/Driver.f90:21-60; Entropy_Viscosity,
+ * EViscosity_GL.f90:19-101; RHS_ent_trans, RHS.f90:21-190), so the state of
+ * a shock run is the pair (J*Q, lagged states) and its adjoint the pair
+ * (Qbar, Slagbar).
+ * dsem_ev_vjp: at the context's state and lagged states, JQbar =
+ *   (d mu_h / d JQ)^T muhbar and Slagbar = (d mu_h / d Slag)^T muhbar; host
+ *   buffers muhbar [K][3][nc*ncg^2], JQbar [K][5][ncg^3], Slagbar
+ *   [K][3][5][nc*ncg^2].  The clamps take the branch the forward took.
+ * dsem_rk_step_vjp_tape_lagged: the adjoint of the step recorded in `slot`
+ *   (dsem_rk_step_record keeps its mu_h and the lagged states it read);
+ *   dQbar and dSlagbar (device) hold on entry the adjoints of the state and
+ *   of the lagged states after the step, on return those before it.
+ * dsem_get/set_lagged_states_device: device copies for checkpoints; valid
+ *   receives whether the context holds lagged states (a fresh start does
+ *   not); set with NULL makes the next step a fresh start (mu_h = 0). */
+int  dsem_ev_vjp(dsem_ctx *ctx, const double *muhbar, double *JQbar, double *Slagbar);
+int  dsem_rk_step_vjp_tape_lagged(dsem_ctx *ctx, double dt, int slot, double *dQbar, double *dSlagbar, double *nubar);
+int  dsem_get_lagged_states_device(dsem_ctx *ctx, double *dstag, int32_t *valid);
+int  dsem_set_lagged_states_device(dsem_ctx *ctx, const double *dstag);
+
 /* modalfilter (src/Energydissipation/energydissipation.f90:720-959; the
  * driver calls it every 50 steps before tstep, src/3DDriver_parfmdf.f90:462-463)
  * on the context's state: per element the momentum J*Q times jacob(1,1,1)

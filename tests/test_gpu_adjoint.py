@@ -51,6 +51,33 @@ def test_rhs_vjp_matches_central_differences(metrics):
             assert _rel(ad, fd) < 1e-6
 
 
+@pytest.mark.parametrize("p", [5, 6, 7, 8])
+def test_rhs_vjp_high_orders(p):
+    """Orders 7 and 8 run the forward through the persistent volume kernel
+    (1024 work-items, next element copied into LDS); the adjoint must agree
+    with central differences there too, and at 5, 6 beside them."""
+    print(f"\n[case] rhs vjp distorted 2^3 p={p}", flush=True)
+    mesh = distorted_box(2, p, amp=0.05, seed=p)
+    _, JQ = _state(mesh, 20 + p)
+    rng = np.random.default_rng(p)
+    with DSEM(mesh, re=400.0) as s:
+        s.set_state(JQ)
+        for trial in range(2):
+            lam = rng.standard_normal(JQ.shape)
+            v = rng.standard_normal(JQ.shape) * np.abs(JQ).mean(axis=(0, 2), keepdims=True)
+            jb, _ = s.rhs_vjp(lam)
+            h = 1e-6
+            s.set_state(JQ + h * v)
+            rp = s.rhs()
+            s.set_state(JQ - h * v)
+            rm = s.rhs()
+            s.set_state(JQ)
+            fd = float(np.sum(lam * (rp - rm)) / (2 * h))
+            ad = float(np.sum(jb * v))
+            print(f"trial {trial}: adjoint {ad:.12e} fd {fd:.12e} rel {_rel(ad, fd):.2e}", flush=True)
+            assert _rel(ad, fd) < 1e-6
+
+
 def test_rhs_vjp_field_by_field():
     """Directions confined to one conserved field at a time, so that no
     field's adjoint can hide behind another's."""
