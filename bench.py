@@ -67,10 +67,17 @@ def kernel_work(n_elem: int, nmort: int, p: int) -> dict:
                          + Mm * nf * d,
                          flops=K * 180 * nc * ncg ** 3,
                          pointwise_flops=K * 3 * ns * 260)
-    # 128 B per face point: both viscous traces and the norm in, the average out
-    w["k_face_visc"] = dict(bytes=Mm * nf * (10 + 1 + 5) * d, flops=Mm * nf * 5 * 6)
-    # 40 B per DOF: Qtp, G, JQ in; G, JQ out
-    w["k_update"] = dict(bytes=K * npt * 5 * (1 + 2 + 2) * d, flops=K * npt * 5 * 16)
+    # 128 B per face point: both viscous traces and the norm in, the average out.
+    # The kernel also folds in send_flux_to_faces_add: it reads the Roe flux
+    # and writes FI - average/Re in place of the average (fused: 168 B).
+    w["k_face_visc"] = dict(bytes=Mm * nf * (10 + 1 + 5) * d, flops=Mm * nf * 5 * 6,
+                            fused_bytes=Mm * nf * (10 + 1 + 5 + 5) * d)
+    # 40 B per DOF: Qtp, G, JQ in; G, JQ out.  The fused kernel also reads the
+    # six faces' total fluxes (face part of the divergence) and 1/J, and
+    # writes the next stage's face traces; G is not read by the first stage
+    # (one launch in five of an RK step).
+    w["k_update"] = dict(bytes=K * npt * 5 * (1 + 2 + 2) * d, flops=K * npt * 5 * 16,
+                         fused_bytes=K * (npt * 5 * (1 + 0.8 + 1 + 2) + npt + 2 * 6 * 5 * nf) * d)
     return w
 
 
@@ -240,6 +247,12 @@ def main():
                          "TFLOP/s": round(wk["flops"] / (avg * 1e-3) / 1e12, 2)}
             if "pointwise_flops" in wk:
                 per[name]["pointwise_TFLOP/s_estimate"] = round(wk["pointwise_flops"] / (avg * 1e-3) / 1e12, 2)
+            if "fused_bytes" in wk:
+                # every array the fused kernel must move once (the per-unit
+                # figure above counts the unfused step it replaces)
+                per[name]["fused_GB"] = round(wk["fused_bytes"] / 1e9, 4)
+                per[name]["fused_frac_of_hbm_peak"] = round(
+                    wk["fused_bytes"] / (avg * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
         dom = max(per, key=lambda k: per[k]["avg_ms"])
         wk = work[dom]
         avg_s = per[dom]["avg_ms"] * 1e-3
