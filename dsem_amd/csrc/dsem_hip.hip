@@ -2420,7 +2420,7 @@ int run_rhs_vjp(dsem_ctx *c, double *JQ, FwdTape f, bool fwd, double *Gb, double
         k_adj_update<P><<<K, BLOCK, 0, s>>>(Gb, Qb, cdt, c->aFTbe);
         k_adj_face_visc<P><<<face_blocks, BLOCK, 0, s>>>(c->aFTbe, f.FV, m, ph, c->aFIb, c->aFVb, nu_f);
         k_adj_volume<P, (P >= 4)><<<K, ABLOCK, 0, s>>>(JQ, m, ph, f.QA, muh, Gb, c->aFVb, c->aQVb, c->aGVb, nu_v, MuB, SlagB);
-        if constexpr (P >= 7) k_adj_gather_dma<P><<<K, GBLOCK, 0, s>>>(m, c->aQVb, c->aGVb, c->aJQvb, c->aQAbe);
+        if constexpr (P >= 5) k_adj_gather_dma<P><<<K, GBLOCK, 0, s>>>(m, c->aQVb, c->aGVb, c->aJQvb, c->aQAbe);
         else k_adj_gather<P><<<K, GBLOCK, 0, s>>>(m, c->aQVb, c->aGVb, c->aJQvb, c->aQAbe);
         k_adj_face_inv<P><<<face_blocks, BLOCK, 0, s>>>(f.T, m, ph, c->aFIb, c->aQAbe, c->aTb);
         k_adj_traces<P><<<K, BLOCK, 0, s>>>(c->aTb, c->aJQvb, m, Out, Qb, Gb, a);
