@@ -10,8 +10,10 @@
                ``tape`` slots (dsem_rk_step_record stores a step's stage
                states and face intermediates; dsem_rk_step_vjp_tape runs its
                adjoint).  The forward pass keeps the snapshots the schedule
-               places on its way; the backward pass restores, advances,
-               snapshots and records/reverses as the schedule says.  The
+               places on its way and records the last steps, which the
+               reversal takes first, on the tape as it takes them; the
+               backward pass restores, advances, snapshots and
+               records/reverses as the schedule says.  The
                default (tape = every, snaps = ceil(nsteps/every) - 1) is the
                uniform schedule: a snapshot every ``every`` steps and one
                extra forward pass; fewer slots trade memory for
@@ -73,16 +75,26 @@ class RKSolve(torch.autograd.Function):
             snaps = -(-nsteps // every) - 1
         plan = Plan(nsteps, max(0, int(snaps)), tape)
         keep = set(plan.forward_snapshots())
+        # The reversal starts with the solve's last steps [tail, nsteps): the
+        # forward pass records them on the tape as it takes them, so their
+        # recording pass (and the restore/advance that would precede it) is
+        # not repeated.
+        tail = plan.tail()
         ev = bool(solver.params["shock"])
         torch.cuda.synchronize()
+        solver.tape_reserve(plan.tape)
         solver.set_state_device(JQ0.data_ptr())      # a fresh start: the first step's mu_h is zero
         held = {0: _snapshot(solver, JQ0.detach().clone(), ev, initial=True)}
         for s in range(1, nsteps + 1):
-            solver.steps(1, dt, check=False)
+            if s - 1 >= tail:
+                solver.rk_step_record(dt, s - 1 - tail)
+            else:
+                solver.steps(1, dt, check=False)
             if s in keep:
                 held[s] = _snapshot(solver, None, ev)
         out = torch.empty_like(JQ0)
         solver.get_state_device(out.data_ptr())
+        ctx.token = solver.tape_owner = object()     # the tail on the tape is this solve's
         ctx.solver, ctx.dt, ctx.plan, ctx.held, ctx.ev = solver, dt, plan, held, ev
         return out
 
@@ -97,7 +109,10 @@ class RKSolve(torch.autograd.Function):
         torch.cuda.synchronize()
         nubar = 0.0
         solver.tape_reserve(plan.tape)
-        for act in plan.actions():
+        # another recording since the forward pass (a second solve on the
+        # same context) replaced the tail: then it is recorded again
+        acts = plan.backward_actions() if getattr(solver, "tape_owner", None) is ctx.token else plan.actions()
+        for act in acts:
             kind = act[0]
             if kind == "restore":
                 _restore(solver, held[act[1]], ev)
@@ -107,10 +122,12 @@ class RKSolve(torch.autograd.Function):
                 held[act[1]] = _snapshot(solver, None, ev)
             elif kind == "free":
                 del held[act[1]]
-            else:   # reverse: record k steps from the current state, then their adjoints backwards
+            else:   # reverse: record k steps from the current state (unless the forward
+                    # pass did), then their adjoints backwards
                 k = act[2]
-                for i in range(k):
-                    solver.rk_step_record(dt, i)
+                if kind == "reverse":
+                    for i in range(k):
+                        solver.rk_step_record(dt, i)
                 for i in reversed(range(k)):
                     if ev:
                         nubar += solver.rk_step_vjp_tape_lagged(dt, i, qbar.data_ptr(), slagbar.data_ptr())

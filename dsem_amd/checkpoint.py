@@ -126,6 +126,24 @@ class Plan:
         return acts
 
 
+    def tail(self) -> int:
+        """First step of the chunk the reversal takes first, [tail, n): the
+        frontier's last chunk, which the forward pass can record on the tape
+        as it takes those steps."""
+        return next(a[1] for a in self.actions() if a[0] == "reverse")
+
+    def backward_actions(self):
+        """actions() for a forward pass that recorded [tail, n) on the tape:
+        the actions that only position the state for that recording are
+        dropped (the frontier places no snapshot before its last chunk, so
+        they are restores and advances), and its reverse becomes
+        ("reverse_recorded", tail, k)."""
+        acts = self.actions()
+        first = next(i for i, a in enumerate(acts) if a[0] == "reverse")
+        assert all(a[0] in ("restore", "advance") for a in acts[:first])
+        return [("reverse_recorded",) + tuple(acts[first][1:])] + acts[first + 1:]
+
+
 def binomial_bound(t: int, s: int) -> int:
     """Griewank's least number of forward steps to reverse t steps with s
     snapshots besides the start state and single-step reversal (tape = 1)."""

@@ -294,9 +294,13 @@ class DSEM:
         return nb.value
 
     def tape_reserve(self, nslots: int) -> None:
+        if nslots > getattr(self, "_tape_slots", 0):
+            self.tape_owner = None  # a larger tape is a new allocation
+            self._tape_slots = nslots
         self._check(self._L.dsem_tape_reserve(self._h, nslots))
 
     def rk_step_record(self, dt: float, slot: int) -> None:
+        self.tape_owner = None      # whoever recorded the tape's contents before no longer owns them
         self._check(self._L.dsem_rk_step_record(self._h, dt, slot))
 
     def rk_step_vjp_tape(self, dt: float, slot: int, qbar_ptr: int) -> float:

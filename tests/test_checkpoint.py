@@ -8,6 +8,9 @@
   `snaps` snapshots, and advances exactly the planned number of steps.
 * tape = every, snaps = n / every - 1 is the uniform schedule: no
   recomputation beyond the recording passes.
+* With the forward pass recording the last chunk (Plan.tail), the backward
+  actions still reverse every step once from its own state, and record and
+  advance no more than the full schedule.
 """
 import pytest
 
@@ -65,3 +68,42 @@ def test_uniform_schedule_is_recovered():
 def test_fewer_snapshots_cost_recomputation():
     costs = [Plan(50, s, 1).recomputed_steps() for s in range(0, 8)]
     assert all(a > b for a, b in zip(costs, costs[1:]))
+
+
+@pytest.mark.parametrize("n,snaps,tape", [(1, 0, 1), (7, 0, 1), (12, 2, 2), (50, 4, 10), (50, 5, 1), (50, 2, 3),
+                                          (37, 3, 4), (100, 6, 2), (50, 0, 10), (10, 1, 5)])
+def test_forward_recorded_tail(n, snaps, tape):
+    """The machine's forward pass records steps tail .. n-1 on the tape."""
+    P = Plan(n, snaps, tape)
+    tail = P.tail()
+    assert 1 <= n - tail <= tape
+    held = {0: 0}
+    for a in P.forward_snapshots():
+        held[a] = a
+    assert all(a <= tail for a in held)
+    tape_steps = list(range(tail, n))       # recorded by the forward pass
+    cur, advanced, recorded, reversed_ = None, 0, 0, []
+    for act in P.backward_actions():
+        if act[0] == "restore":
+            cur = held[act[1]]
+        elif act[0] == "advance":
+            cur += act[1]
+            advanced += act[1]
+        elif act[0] == "snapshot":
+            assert cur == act[1]
+            held[act[1]] = cur
+            assert len(held) - 1 <= snaps
+        elif act[0] == "free":
+            del held[act[1]]
+        elif act[0] in ("reverse", "reverse_recorded"):
+            a, k = act[1], act[2]
+            if act[0] == "reverse":
+                assert cur == a and 1 <= k <= tape
+                tape_steps = list(range(a, a + k))
+                recorded += k
+            assert tape_steps == list(range(a, a + k))
+            reversed_ += list(range(a + k - 1, a - 1, -1))
+            cur = None
+    assert reversed_ == list(range(n - 1, -1, -1))
+    assert advanced <= P.recomputed_steps()
+    assert recorded == tail

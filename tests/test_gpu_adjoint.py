@@ -274,3 +274,31 @@ def test_binomial_schedules_give_the_uniform_gradient():
     for key, (gx, gnu) in grads.items():
         assert np.linalg.norm(gx - ref_x) <= 1e-13 * np.linalg.norm(ref_x), key
         assert abs(gnu - ref_nu) <= 1e-12 * abs(ref_nu), key
+
+
+def test_second_solve_before_backward_keeps_the_gradient():
+    """The forward pass records the solve's last chunk on the context's tape.
+    A second solve on the same context before the first one's backward
+    replaces that recording; the first backward then records it again and
+    its gradient is unchanged."""
+    import torch
+    from dsem_amd import autograd as A
+    print("\n[case] interleaved solves 2^3 p=4, 8 steps", flush=True)
+    mesh = M.periodic_box(2, 4)
+    g, JQ = _state(mesh, 3, perturb=1e-2)
+    _, JQ2 = _state(mesh, 4, perturb=1e-2)
+    dt = 0.4 * O.dtcalc(g, JQ, O.phys(re=800.0))
+    with DSEM(mesh, re=800.0, metrics="per_point") as s:
+        bmat = A.bmat_tensor(s, "cuda")
+
+        def grad(interleave):
+            x = torch.tensor(JQ, device="cuda", requires_grad=True)
+            J = A.kinetic_energy_integral(A.solve(s, x, 8, dt, every=4), bmat)
+            if interleave:
+                y = torch.tensor(JQ2, device="cuda", requires_grad=True)
+                A.kinetic_energy_integral(A.solve(s, y, 8, dt, every=4), bmat)
+            J.backward()
+            return x.grad.cpu().numpy()
+
+        alone, inter = grad(False), grad(True)
+    assert np.linalg.norm(inter - alone) <= 1e-13 * np.linalg.norm(alone)
