@@ -9,6 +9,8 @@ fixture lets the GPU history be compared with it at full size without the
 ~30 CPU-minutes the oracle needs for it.
 
     python tests/golden/make_tgv_history.py [--threads 8] [--steps 200]
+    python tests/golden/make_tgv_history.py --n 8 --p 4 --steps 8266 --every 20 \
+        --out tests/golden/tgv8p4_history.txt     # to t = 10, through the dissipation peak
 """
 import argparse
 import os
@@ -33,6 +35,7 @@ def main():
     ap.add_argument("--p", type=int, default=8)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--threads", type=int, default=8)
+    ap.add_argument("--every", type=int, default=1, help="keep every k-th row (and the last)")
     ap.add_argument("--out", default=os.path.join(HERE, "tgv22p8_history.txt"))
     a = ap.parse_args()
     re, pr, cfl = 1600.0, 0.71, 0.5
@@ -57,6 +60,7 @@ def main():
               f"({time.time() - t0:.0f} s)", flush=True)
     hdr = (f"FP64 oracle TGV history: n={a.n} p={a.p} Re={re} Pr={pr} CFL={cfl} dt={dt!r} variant=isothermal\n"
            "step time Ek -dEk/dt eps")
+    rows = [r for r in rows if r[0] % a.every == 0 or r[0] == a.steps]
     np.savetxt(a.out, np.array(rows), fmt=["%d", "%.17e", "%.17e", "%.17e", "%.17e"], header=hdr)
 
 
