@@ -36,6 +36,48 @@ def distorted_box(n: int, p: int, amp: float = 0.08, seed: int = 11) -> M.Mesh:
     return M.build_topology(p, corners, nodes, periodic=True)
 
 
+def graded_box(nx: int, ny: int, nz: int, p: int, lengths=(2 * math.pi, 3.0, 5.0), ratio: float = 1.6) -> M.Mesh:
+    """Periodic box of nx x ny x nz elements with edge lengths ``lengths`` and
+    geometrically graded spacing along each axis (largest / smallest element
+    = ratio): non-cubic, anisotropic, different counts per direction.
+    Element order as the reference mesh files (y fastest, then x, then z)."""
+    def axis(n, L):
+        w = ratio ** (np.arange(n) / max(n - 1, 1))
+        return np.concatenate([[0.0], np.cumsum(w / w.sum() * L)])
+    xs, ys, zs = axis(nx, lengths[0]), axis(ny, lengths[1]), axis(nz, lengths[2])
+    K = nx * ny * nz
+    corners = np.zeros((K, 8, 3))
+    nodes = np.zeros((K, 8), dtype=np.int64)
+    e = 0
+    for iz in range(nz):
+        for ix in range(nx):
+            for iy in range(ny):
+                cn = [(ix, iy, iz), (ix + 1, iy, iz), (ix + 1, iy + 1, iz), (ix, iy + 1, iz),
+                      (ix, iy, iz + 1), (ix + 1, iy, iz + 1), (ix + 1, iy + 1, iz + 1), (ix, iy + 1, iz + 1)]
+                for a, (i, j, k) in enumerate(cn):
+                    corners[e, a] = (xs[i], ys[j], zs[k])
+                    nodes[e, a] = j + (ny + 1) * (i + (nx + 1) * k)
+                e += 1
+    return M.build_topology(p, corners, nodes, periodic=True)
+
+
+def periodic_state(xg, jacob, lengths, seed=0, amp=0.05):
+    """A smooth state periodic on the box: rho, u, v, w, p as low Fourier
+    modes of the box periods with seeded phases; returned as J*Q [K,5,np]
+    (xg [K,np,3], jacob = 1/J [K,np])."""
+    rng = np.random.default_rng(seed)
+    ph = rng.uniform(0, 2 * np.pi, (5, 3))
+    kx, ky, kz = (2 * np.pi / L for L in lengths)
+    x, y, z = xg[..., 0], xg[..., 1], xg[..., 2]
+    f = [np.sin(kx * x + ph[i, 0]) * np.cos(ky * y + ph[i, 1]) + np.sin(kz * z + ph[i, 2]) for i in range(5)]
+    rho = 1.0 + amp * f[0]
+    u, v, w = 0.3 * f[1], 0.3 * f[2], 0.3 * f[3]
+    pr = 71.4 + amp * 10 * f[4]
+    E = pr / 0.4 + 0.5 * rho * (u * u + v * v + w * w)
+    Q = np.stack([rho, rho * u, rho * v, rho * w, E], axis=1)
+    return Q / jacob[:, None, :]
+
+
 def rel_l2(a, b):
     """Per-field relative L2 error of a against b, arrays [K, 5, np]."""
     num = np.sqrt(((a - b) ** 2).sum(axis=(0, 2)))
