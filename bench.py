@@ -179,8 +179,8 @@ def main():
                     help="run one device-to-device copy kernel of this many bytes first (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (multi-GPU sharding is a "
-                                   "design note only)"}))
+        print(json.dumps({"error": "the library runs on one GPU; --gpus must be 1 (sharding across GPUs is "
+                                   "designed, not built)"}))
         sys.exit(2)
 
     if args.copy_bytes > 0:
