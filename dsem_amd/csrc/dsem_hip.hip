@@ -3230,8 +3230,6 @@ static int vjp_check(dsem_ctx *c, bool allow_shock = false)
                                 "dsem_rk_step_vjp_tape_lagged) or dsem_ev_vjp");
     if (c->params.precision != DSEM_PREC_FP64)
         return set_err(c, DSEM_ERR_UNSUPPORTED, "the adjoint is built for the FP64 path");
-    if (c->rotated)
-        return set_err(c, DSEM_ERR_UNSUPPORTED, "the adjoint covers meshes whose conforming faces are all aligned");
     HIP_TRY(c, hipSetDevice(c->device));
     return adjoint_alloc(c);
 }

@@ -78,8 +78,7 @@ typedef struct dsem_ctx dsem_ctx;
  * (src/Grid_Routines/face_operations.f90:404-460): DFLT, B1F2, B1B2, F1B2,
  * F2F1, B2F1, B2B1, F2B1 = 0..7 (element_library.f90:99-101); mortar point
  * (i,j) takes the slave face point that rotate_to_mortar
- * (src/Parabolic3D/mortar_operations.f90:461-605) maps to it.  The adjoint
- * entry points need every orientation to be 0. */
+ * (src/Parabolic3D/mortar_operations.f90:461-605) maps to it. */
 typedef struct {
     int32_t K;                     /* elements                               */
     int32_t p;                     /* order: ncg = p+1 Gauss, nc = p+2 Lobatto, 1..8 */
