@@ -80,32 +80,34 @@ class RKSolve(torch.autograd.Function):
         # recording pass (and the restore/advance that would precede it) is
         # not repeated.
         tail = plan.tail()
-        ev = bool(solver.params["shock"])
+        # entropy viscosity and the Smagorinsky model read the lagged
+        # staggered states: snapshots and the adjoint state carry them
+        lagged = bool(solver.params["shock"]) or solver.les
         torch.cuda.synchronize()
         solver.tape_reserve(plan.tape)
         solver.set_state_device(JQ0.data_ptr())      # a fresh start: the first step's mu_h is zero
-        held = {0: _snapshot(solver, JQ0.detach().clone(), ev, initial=True)}
+        held = {0: _snapshot(solver, JQ0.detach().clone(), lagged, initial=True)}
         for s in range(1, nsteps + 1):
             if s - 1 >= tail:
                 solver.rk_step_record(dt, s - 1 - tail)
             else:
                 solver.steps(1, dt, check=False)
             if s in keep:
-                held[s] = _snapshot(solver, None, ev)
+                held[s] = _snapshot(solver, None, lagged)
         out = torch.empty_like(JQ0)
         solver.get_state_device(out.data_ptr())
         ctx.token = solver.tape_owner = object()     # the tail on the tape is this solve's
-        ctx.solver, ctx.dt, ctx.plan, ctx.held, ctx.ev = solver, dt, plan, held, ev
+        ctx.solver, ctx.dt, ctx.plan, ctx.held, ctx.lagged = solver, dt, plan, held, lagged
         return out
 
     @staticmethod
     def backward(ctx, gout: torch.Tensor):
-        solver, dt, plan, held, ev = ctx.solver, ctx.dt, ctx.plan, ctx.held, ctx.ev
+        solver, dt, plan, held, lagged = ctx.solver, ctx.dt, ctx.plan, ctx.held, ctx.lagged
         qbar = gout.detach().to(torch.float64).contiguous().clone()
-        # with entropy viscosity the adjoint state also carries the lagged
-        # staggered states' adjoint (zero after the last step: J reads the
-        # state only)
-        slagbar = torch.zeros((solver.K, 3, 5, solver.ns), dtype=torch.float64, device=qbar.device) if ev else None
+        # with entropy viscosity or the Smagorinsky model the adjoint state
+        # also carries the lagged staggered states' adjoint (zero after the
+        # last step: J reads the state only)
+        slagbar = torch.zeros((solver.K, 3, 5, solver.ns), dtype=torch.float64, device=qbar.device) if lagged else None
         torch.cuda.synchronize()
         nubar = 0.0
         solver.tape_reserve(plan.tape)
@@ -115,11 +117,11 @@ class RKSolve(torch.autograd.Function):
         for act in acts:
             kind = act[0]
             if kind == "restore":
-                _restore(solver, held[act[1]], ev)
+                _restore(solver, held[act[1]], lagged)
             elif kind == "advance":
                 solver.steps(act[1], dt, check=False)
             elif kind == "snapshot":
-                held[act[1]] = _snapshot(solver, None, ev)
+                held[act[1]] = _snapshot(solver, None, lagged)
             elif kind == "free":
                 del held[act[1]]
             else:   # reverse: record k steps from the current state (unless the forward
@@ -129,7 +131,7 @@ class RKSolve(torch.autograd.Function):
                     for i in range(k):
                         solver.rk_step_record(dt, i)
                 for i in reversed(range(k)):
-                    if ev:
+                    if lagged:
                         nubar += solver.rk_step_vjp_tape_lagged(dt, i, qbar.data_ptr(), slagbar.data_ptr())
                     else:
                         nubar += solver.rk_step_vjp_tape(dt, i, qbar.data_ptr())
@@ -137,13 +139,14 @@ class RKSolve(torch.autograd.Function):
         return qbar, torch.tensor(nubar, dtype=torch.float64, device=qbar.device), None, None, None, None, None, None
 
 
-def _snapshot(solver: DSEM, state, ev: bool, initial: bool = False):
-    """The solver's current state (or the given tensor) and, with entropy
-    viscosity, its lagged staggered states (None on a fresh start)."""
+def _snapshot(solver: DSEM, state, lagged: bool, initial: bool = False):
+    """The solver's current state (or the given tensor) and, when a model
+    reads them (entropy viscosity, Smagorinsky), its lagged staggered states
+    (None on a fresh start)."""
     if state is None:
         state = torch.empty((solver.K, 5, solver.np), dtype=torch.float64, device="cuda")
         solver.get_state_device(state.data_ptr())
-    if not ev:
+    if not lagged:
         return state
     lag = None
     if not initial:
@@ -153,8 +156,8 @@ def _snapshot(solver: DSEM, state, ev: bool, initial: bool = False):
     return (state, lag)
 
 
-def _restore(solver: DSEM, snap, ev: bool) -> None:
-    if not ev:
+def _restore(solver: DSEM, snap, lagged: bool) -> None:
+    if not lagged:
         solver.set_state_device(snap.data_ptr())
         return
     state, lag = snap

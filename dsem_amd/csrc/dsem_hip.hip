@@ -37,6 +37,7 @@
 //              (per element, affine meshes) -- selected by the stride ms.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -2051,7 +2052,12 @@ struct dsem_ctx {
     double *aTb = nullptr, *aFIb = nullptr, *aJQvb = nullptr, *aNuPart = nullptr, *aNu = nullptr;
     double *aGb = nullptr, *aLam = nullptr, *aOut = nullptr;
     double *aMuB = nullptr, *aWB = nullptr, *aSlagB = nullptr;   // entropy-viscosity adjoint (shock = 1)
-    std::vector<char> tape_lag;                // per tape slot: the recorded step had lagged states
+    double *aNutB = nullptr;                   // Smagorinsky adjoint: that of the staggered nu_t
+    // per tape slot: bit 0 the recorded step had lagged states, bit 1 it ran
+    // the Smagorinsky model (with the constants in tape_cs, tape_numax)
+    std::vector<char> tape_lag;
+    std::vector<double> tape_cs, tape_numax;
+    bool tape_les = false;                     // the tape's slots hold nu_t (the layout of tape_reserve)
     // step tape: per slot and stage the stage input state and the forward
     // intermediates the adjoint reads (face traces T, mortar averages QA,
     // viscous face traces FV)
@@ -2379,11 +2385,18 @@ int adjoint_alloc(dsem_ctx *c)
     TRY(dev_alloc(c, &c->aGb, n));
     TRY(dev_alloc(c, &c->aLam, n));
     TRY(dev_alloc(c, &c->aOut, n));
-    if (c->params.shock) {
-        TRY(dev_alloc(c, &c->aMuB, K * 3 * c->ns));
-        TRY(dev_alloc(c, &c->aWB, K * 18 * c->np));
-        TRY(dev_alloc(c, &c->aSlagB, K * 3 * NEQ * c->ns));
-    }
+    return DSEM_OK;
+}
+
+// Buffers of the lagged-state adjoints: entropy viscosity (fixed at
+// creation) and the Smagorinsky model (switched on at any time).
+int adjoint_lag_alloc(dsem_ctx *c)
+{
+    const size_t K = c->K;
+    if (c->params.shock && !c->aMuB) TRY(dev_alloc(c, &c->aMuB, K * 3 * c->ns));
+    if ((c->params.shock || c->les) && !c->aWB) TRY(dev_alloc(c, &c->aWB, K * (c->params.shock ? 18 : 12) * c->np));
+    if ((c->params.shock || c->les) && !c->aSlagB) TRY(dev_alloc(c, &c->aSlagB, K * 3 * NEQ * c->ns));
+    if (c->les && !c->aNutB) TRY(dev_alloc(c, &c->aNutB, K * 3 * c->ns));
     return DSEM_OK;
 }
 
@@ -2400,9 +2413,12 @@ struct FwdTape {
 // With entropy viscosity: muh is the step's mu_h (staggered), MuB
 // accumulates its adjoint, and SlagB (the adjoint of the lagged states this
 // evaluation leaves, last stage only) joins the staggered-state adjoints.
+// With the Smagorinsky model: nut is the step's staggered nu_t and NutB
+// accumulates its adjoint.
 template <int P>
 int run_rhs_vjp(dsem_ctx *c, double *JQ, FwdTape f, bool fwd, double *Gb, double *Qb, double cdt, double a,
-                double *Out, double *nu_slot, const double *muh, double *MuB, const double *SlagB)
+                double *Out, double *nu_slot, const double *muh, double *MuB, const double *SlagB,
+                const double *nut, double *NutB)
 {
     using S = Sz<P>;
     const DevMesh m = dev_mesh(c);
@@ -2419,7 +2435,8 @@ int run_rhs_vjp(dsem_ctx *c, double *JQ, FwdTape f, bool fwd, double *Gb, double
     return launch(c, KID_ADJ, [&] {
         k_adj_update<P><<<K, BLOCK, 0, s>>>(Gb, Qb, cdt, c->aFTbe);
         k_adj_face_visc<P><<<face_blocks, BLOCK, 0, s>>>(c->aFTbe, f.FV, m, ph, c->aFIb, c->aFVb, nu_f);
-        k_adj_volume<P, (P >= 4)><<<K, ABLOCK, 0, s>>>(JQ, m, ph, f.QA, muh, Gb, c->aFVb, c->aQVb, c->aGVb, nu_v, MuB, SlagB);
+        k_adj_volume<P, (P >= 4)><<<K, ABLOCK, 0, s>>>(JQ, m, ph, f.QA, muh, Gb, c->aFVb, c->aQVb, c->aGVb, nu_v, MuB, SlagB,
+                                                        nut, NutB);
         if constexpr (P >= 5) k_adj_gather_dma<P><<<K, GBLOCK, 0, s>>>(m, c->aQVb, c->aGVb, c->aJQvb, c->aQAbe);
         else k_adj_gather<P><<<K, GBLOCK, 0, s>>>(m, c->aQVb, c->aGVb, c->aJQvb, c->aQAbe);
         k_adj_face_inv<P><<<face_blocks, BLOCK, 0, s>>>(f.T, m, ph, c->aFIb, c->aQAbe, c->aTb);
@@ -2429,9 +2446,9 @@ int run_rhs_vjp(dsem_ctx *c, double *JQ, FwdTape f, bool fwd, double *Gb, double
 
 int rhs_vjp_dispatch(dsem_ctx *c, double *JQ, FwdTape f, bool fwd, double *Gb, double *Qb, double cdt, double a,
                      double *Out, double *nu_slot, const double *muh = nullptr, double *MuB = nullptr,
-                     const double *SlagB = nullptr)
+                     const double *SlagB = nullptr, const double *nut = nullptr, double *NutB = nullptr)
 {
-    DISPATCH_P(c, run_rhs_vjp, JQ, f, fwd, Gb, Qb, cdt, a, Out, nu_slot, muh, MuB, SlagB);
+    DISPATCH_P(c, run_rhs_vjp, JQ, f, fwd, Gb, Qb, cdt, a, Out, nu_slot, muh, MuB, SlagB, nut, NutB);
 }
 
 EvArgs ev_args(const dsem_ctx *c)
@@ -2466,6 +2483,28 @@ int ev_vjp_dispatch(dsem_ctx *c, const double *JQ, const double *Slag, const dou
     DISPATCH_P(c, run_ev_vjp, JQ, Slag, MuB, JQb, SlagB);
 }
 
+// Adjoint of the Smagorinsky update (k_smag, no sensors) at state JQ and
+// lagged states Slag: JQb += (d nu_t / d JQ)^T NutB, SlagB += (d nu_t /
+// d Slag)^T NutB, nu_slot = <NutB, d nu_t / d nu> (nu_t is linear in Re).
+template <int P>
+int run_les_vjp(dsem_ctx *c, const double *JQ, const double *Slag, double cs, double numax, const double *NutB,
+                double *JQb, double *SlagB, double *nu_slot)
+{
+    const DevMesh m = dev_mesh(c);
+    const DevPhys ph = dev_phys(c);
+    hipStream_t s = c->stream;
+    return launch(c, KID_ADJ, [&] {
+        k_adj_smag<P><<<c->K, BLOCK, 0, s>>>(JQ, Slag, m, ph, cs, numax, NutB, JQb, c->aWB, c->aNuPart);
+        k_adj_smag_gather<P><<<c->K, BLOCK, 0, s>>>(c->aWB, SlagB, c->aNuPart, c->K, nu_slot);
+    });
+}
+
+int les_vjp_dispatch(dsem_ctx *c, const double *JQ, const double *Slag, double cs, double numax, const double *NutB,
+                     double *JQb, double *SlagB, double *nu_slot)
+{
+    DISPATCH_P(c, run_les_vjp, JQ, Slag, cs, numax, NutB, JQb, SlagB, nu_slot);
+}
+
 // Tape layout per slot and stage: state [n], T [nT], QA [nQA], FV [nT].
 size_t tape_stage_size(const dsem_ctx *c)
 {
@@ -2474,11 +2513,14 @@ size_t tape_stage_size(const dsem_ctx *c)
     return n + 2 * nT + nQA;
 }
 
-// Entropy viscosity: after a slot's five stages, the step's mu_h
-// [K][3][ns] and the lagged states it was computed from [K][3][5][ns].
+// After a slot's five stages: with entropy viscosity the step's mu_h
+// [K][3][ns]; with entropy viscosity or the Smagorinsky model the lagged
+// states the step read [K][3][5][ns]; with the Smagorinsky model its nu_t
+// [K][3][ns].
 size_t tape_ev_size(const dsem_ctx *c)
 {
-    return c->params.shock ? (size_t)c->K * 3 * c->ns * (1 + NEQ) : 0;
+    const size_t nm = (size_t)c->K * 3 * c->ns;
+    return (c->params.shock ? nm : 0) + ((c->params.shock || c->tape_les) ? nm * NEQ : 0) + (c->tape_les ? nm : 0);
 }
 
 size_t tape_slot_size(const dsem_ctx *c)
@@ -2486,11 +2528,15 @@ size_t tape_slot_size(const dsem_ctx *c)
     return 5 * tape_stage_size(c) + tape_ev_size(c);
 }
 
-void tape_ev(const dsem_ctx *c, int slot, double **muh, double **slag)
+void tape_ev(const dsem_ctx *c, int slot, double **muh, double **slag, double **nut)
 {
+    const size_t nm = (size_t)c->K * 3 * c->ns;
     double *b = c->tape + (size_t)slot * tape_slot_size(c) + 5 * tape_stage_size(c);
-    *muh = b;
-    *slag = b + (size_t)c->K * 3 * c->ns;
+    *muh = c->params.shock ? b : nullptr;
+    if (c->params.shock) b += nm;
+    *slag = (c->params.shock || c->tape_les) ? b : nullptr;
+    b += (c->params.shock || c->tape_les) ? nm * NEQ : 0;
+    *nut = c->tape_les ? b : nullptr;
 }
 
 void tape_stage(const dsem_ctx *c, int slot, int k, double **S, FwdTape *f)
@@ -2506,7 +2552,8 @@ void tape_stage(const dsem_ctx *c, int slot, int k, double **S, FwdTape *f)
 
 int tape_reserve(dsem_ctx *c, int nslots)
 {
-    if (nslots <= c->tape_slots) return DSEM_OK;
+    if (nslots <= c->tape_slots && c->tape_les == c->les) return DSEM_OK;
+    nslots = std::max(nslots, c->tape_slots);   // a layout change keeps the slot count
     if (c->tape) {
         HIP_TRY(c, hipStreamSynchronize(c->stream));
         HIP_TRY(c, hipFree(c->tape));
@@ -2515,9 +2562,12 @@ int tape_reserve(dsem_ctx *c, int nslots)
         c->tape = nullptr;
         c->tape_slots = 0;
     }
+    c->tape_les = c->les;
     TRY(dev_alloc(c, &c->tape, (size_t)nslots * tape_slot_size(c)));
     c->tape_slots = nslots;
     c->tape_lag.assign(nslots, 0);
+    c->tape_cs.assign(nslots, 0.0);
+    c->tape_numax.assign(nslots, 0.0);
     return DSEM_OK;
 }
 
@@ -2529,18 +2579,32 @@ int record_step(dsem_ctx *c, double dt, int slot)
     const size_t n = (size_t)c->K * NEQ * c->np;
     double *T0 = c->T, *QA0 = c->QA, *FV0 = c->FV;
     int rc = DSEM_OK;
-    if (c->params.shock) {
-        // the step's entropy viscosity (tstep's Entropy call) and the lagged
-        // states it reads, kept for the adjoint
-        double *tm, *ts;
-        tape_ev(c, slot, &tm, &ts);
-        c->tape_lag[slot] = c->have_lag ? 1 : 0;
+    if (c->les != c->tape_les)
+        return set_err(c, DSEM_ERR_ARG, "the Smagorinsky switch changed since dsem_tape_reserve: reserve again");
+    const bool lag_model = c->params.shock || c->les;
+    c->tape_lag[slot] = 0;
+    if (lag_model) {
+        // the step's entropy viscosity (tstep's Entropy call), eddy viscosity
+        // (its Smagorinsky call) and the lagged states they read, kept for the
+        // adjoint
+        double *tm, *ts, *tn;
+        tape_ev(c, slot, &tm, &ts, &tn);
+        c->tape_lag[slot] = (c->have_lag ? 1 : 0) | (c->les ? 2 : 0);
+        c->tape_cs[slot] = c->les_cs;
+        c->tape_numax[slot] = c->les_numax;
         if (c->have_lag)
             HIP_TRY(c, hipMemcpyAsync(ts, c->Slag, sizeof(double) * (size_t)c->K * 3 * NEQ * c->ns,
                                       hipMemcpyDeviceToDevice, c->stream));
-        TRY(ev_step(c, false));
-        HIP_TRY(c, hipMemcpyAsync(tm, c->muh, sizeof(double) * (size_t)c->K * 3 * c->ns, hipMemcpyDeviceToDevice,
-                                  c->stream));
+        if (c->params.shock) {
+            TRY(ev_step(c, false));
+            HIP_TRY(c, hipMemcpyAsync(tm, c->muh, sizeof(double) * (size_t)c->K * 3 * c->ns,
+                                      hipMemcpyDeviceToDevice, c->stream));
+        }
+        if (c->les) {
+            TRY(les_step(c, false));
+            HIP_TRY(c, hipMemcpyAsync(tn, c->nut, sizeof(double) * (size_t)c->K * 3 * c->ns,
+                                      hipMemcpyDeviceToDevice, c->stream));
+        }
     }
     for (int k = 0; k < 5 && rc == DSEM_OK; ++k) {
         double *S;
@@ -2559,12 +2623,12 @@ int record_step(dsem_ctx *c, double dt, int slot)
         c->QA = f.QA;
         c->FV = f.FV;
         // the last stage leaves the next step's lagged states, as dsem_rk_steps does
-        rc = rhs_dispatch(c, c->JQ, st, nullptr, false, false, c->params.shock && k == 4);
+        rc = rhs_dispatch(c, c->JQ, st, nullptr, false, false, (lag_model || c->keep_lag) && k == 4);
         c->T = T0;
         c->QA = QA0;
         c->FV = FV0;
     }
-    if (rc == DSEM_OK && c->params.shock) c->have_lag = true;
+    if (rc == DSEM_OK && (lag_model || c->keep_lag)) c->have_lag = true;
     c->traces_cur = false;
     return rc;
 }
@@ -2572,49 +2636,49 @@ int record_step(dsem_ctx *c, double dt, int slot)
 // Adjoint of the step recorded in `slot`: dQbar (device) from the adjoint of
 // the state after the step to that of the state before it; nu_host += the
 // step's <., dR/dnu>.
-// With entropy viscosity, dSlagbar (device, nullable) holds on entry the
-// adjoint of the lagged states the step leaves (its last stage's
-// face-averaged staggered states) and on return that of the lagged states
-// the step's mu_h was computed from; mu_h's own adjoint, accumulated over
-// the five stages, goes back through k_ev's adjoint into dQbar (mu_h is a
-// function of the step's start state) and dSlagbar.
+// With entropy viscosity or the Smagorinsky model, dSlagbar (device,
+// nullable) holds on entry the adjoint of the lagged states the step leaves
+// (its last stage's face-averaged staggered states) and on return that of
+// the lagged states the step's mu_h and nu_t were computed from; their own
+// adjoints, accumulated over the five stages, go back through k_ev's and
+// k_smag's adjoints into dQbar (both are functions of the step's start
+// state) and dSlagbar.
 int vjp_recorded_step(dsem_ctx *c, double dt, int slot, double *dQbar, double *nu_host, double *dSlagbar = nullptr)
 {
-    const size_t n = (size_t)c->K * NEQ * c->np;
+    const size_t n = (size_t)c->K * NEQ * c->np, nm = (size_t)c->K * 3 * c->ns;
     const bool ev = c->params.shock;
-    double *tm = nullptr, *ts = nullptr;
-    if (ev) {
-        tape_ev(c, slot, &tm, &ts);
-        HIP_TRY(c, hipMemsetAsync(c->aMuB, 0, sizeof(double) * (size_t)c->K * 3 * c->ns, c->stream));
-    }
+    const bool les = (c->tape_lag[slot] & 2) != 0;
+    const bool lagged = (c->tape_lag[slot] & 1) != 0;
+    double *tm = nullptr, *ts = nullptr, *tn = nullptr;
+    if (ev || les) tape_ev(c, slot, &tm, &ts, &tn);
+    if (ev) HIP_TRY(c, hipMemsetAsync(c->aMuB, 0, sizeof(double) * nm, c->stream));
+    if (les) HIP_TRY(c, hipMemsetAsync(c->aNutB, 0, sizeof(double) * nm, c->stream));
     HIP_TRY(c, hipMemsetAsync(c->aGb, 0, sizeof(double) * n, c->stream));
     for (int k = 4; k >= 0; --k) {
         double *S;
         FwdTape f;
         tape_stage(c, slot, k, &S, &f);
         TRY(rhs_vjp_dispatch(c, S, f, false, c->aGb, dQbar, kCrk[k] * dt, kArk[k], nullptr, c->aNu + 1 + k,
-                             ev ? tm : nullptr, ev ? c->aMuB : nullptr, (ev && k == 4) ? dSlagbar : nullptr));
+                             ev ? tm : nullptr, ev ? c->aMuB : nullptr, ((ev || les) && k == 4) ? dSlagbar : nullptr,
+                             les ? tn : nullptr, les ? c->aNutB : nullptr));
     }
-    if (ev && dSlagbar) {
-        if (c->tape_lag[slot]) {
-            double *S0;
-            FwdTape f0;
-            tape_stage(c, slot, 0, &S0, &f0);
-            TRY(ev_vjp_dispatch(c, S0, ts, c->aMuB, dQbar, dSlagbar));
-        } else {
-            // a fresh start: mu_h = 0 whatever the state
-            HIP_TRY(c, hipMemsetAsync(dSlagbar, 0, sizeof(double) * (size_t)c->K * 3 * NEQ * c->ns, c->stream));
-        }
-    } else if (ev && c->tape_lag[slot]) {
+    if ((ev || les) && lagged) {
         double *S0;
         FwdTape f0;
         tape_stage(c, slot, 0, &S0, &f0);
-        TRY(ev_vjp_dispatch(c, S0, ts, c->aMuB, dQbar, c->aSlagB));
+        double *slb = dSlagbar ? dSlagbar : c->aSlagB;
+        if (ev) TRY(ev_vjp_dispatch(c, S0, ts, c->aMuB, dQbar, slb));   // sets slb
+        else HIP_TRY(c, hipMemsetAsync(slb, 0, sizeof(double) * nm * NEQ, c->stream));
+        if (les)
+            TRY(les_vjp_dispatch(c, S0, ts, c->tape_cs[slot], c->tape_numax[slot], c->aNutB, dQbar, slb, c->aNu + 6));
+    } else if (dSlagbar) {
+        // a fresh start: mu_h = nu_t = 0 whatever the state
+        HIP_TRY(c, hipMemsetAsync(dSlagbar, 0, sizeof(double) * nm * NEQ, c->stream));
     }
-    double h[5];
+    double h[6];
     HIP_TRY(c, hipMemcpyAsync(h, c->aNu + 1, sizeof(h), hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
-    if (nu_host) *nu_host = h[4] + h[3] + h[2] + h[1] + h[0];
+    if (nu_host) *nu_host = h[4] + h[3] + h[2] + h[1] + h[0] + ((les && lagged) ? h[5] : 0.0);
     return DSEM_OK;
 }
 
@@ -3220,18 +3284,22 @@ int dsem_rk_step(dsem_ctx *c, double t, double dt, double *resid_out)
     return dsem_rk_steps(c, t, dt, 1, resid_out);
 }
 
-static int vjp_check(dsem_ctx *c, bool allow_shock = false)
+// allow_lagged: the tape entry points that carry the adjoint of the lagged
+// states (entropy viscosity and the Smagorinsky model read them).
+static int vjp_check(dsem_ctx *c, bool allow_lagged = false)
 {
     if (!c || !c->stream) return set_err(c, DSEM_ERR_ARG, "vjp arguments");
-    if ((c->params.shock && !allow_shock) || c->les)
+    if ((c->params.shock || c->les) && !allow_lagged)
         return set_err(c, DSEM_ERR_UNSUPPORTED,
-                       c->les ? "the adjoint does not cover the Smagorinsky model"
-                              : "with entropy viscosity use the tape entry points (dsem_rk_step_record, "
-                                "dsem_rk_step_vjp_tape_lagged) or dsem_ev_vjp");
+                       "with entropy viscosity or the Smagorinsky model use the tape entry points "
+                       "(dsem_rk_step_record, dsem_rk_step_vjp_tape_lagged) or dsem_ev_vjp");
+    if (c->les && c->les_sensors)
+        return set_err(c, DSEM_ERR_UNSUPPORTED, "the adjoint does not cover the Smagorinsky sensors");
     if (c->params.precision != DSEM_PREC_FP64)
         return set_err(c, DSEM_ERR_UNSUPPORTED, "the adjoint is built for the FP64 path");
     HIP_TRY(c, hipSetDevice(c->device));
-    return adjoint_alloc(c);
+    TRY(adjoint_alloc(c));
+    return adjoint_lag_alloc(c);
 }
 
 int dsem_rhs_vjp_device(dsem_ctx *c, const double *dJQ, const double *dLam, double *dJQbar, double *nubar)
@@ -3294,6 +3362,8 @@ int dsem_rk_step_vjp_tape(dsem_ctx *c, double dt, int slot, double *dQbar, doubl
 {
     TRY(vjp_check(c));
     if (slot < 0 || slot >= c->tape_slots || !dQbar) return set_err(c, DSEM_ERR_ARG, "rk_step_vjp_tape arguments");
+    if (c->tape_lag[slot] & 2)
+        return set_err(c, DSEM_ERR_UNSUPPORTED, "slot recorded with the Smagorinsky model: use dsem_rk_step_vjp_tape_lagged");
     return vjp_recorded_step(c, dt, slot, dQbar, nubar);
 }
 
@@ -3301,8 +3371,10 @@ int dsem_rk_step_vjp_tape_lagged(dsem_ctx *c, double dt, int slot, double *dQbar
 {
     TRY(vjp_check(c, true));
     if (slot < 0 || slot >= c->tape_slots || !dQbar) return set_err(c, DSEM_ERR_ARG, "rk_step_vjp_tape_lagged arguments");
-    if (c->params.shock && !dSlagbar) return set_err(c, DSEM_ERR_ARG, "rk_step_vjp_tape_lagged: dSlagbar needed with shock = 1");
-    return vjp_recorded_step(c, dt, slot, dQbar, nubar, c->params.shock ? dSlagbar : nullptr);
+    const bool lag_model = c->params.shock || (c->tape_lag[slot] & 2);
+    if (lag_model && !dSlagbar)
+        return set_err(c, DSEM_ERR_ARG, "rk_step_vjp_tape_lagged: dSlagbar needed with shock = 1 or the Smagorinsky model");
+    return vjp_recorded_step(c, dt, slot, dQbar, nubar, lag_model ? dSlagbar : nullptr);
 }
 
 int dsem_ev_vjp(dsem_ctx *c, const double *muhbar, double *JQbar, double *Slagbar)

@@ -68,6 +68,7 @@ class DSEM:
         self.np, self.ns = self.ncg ** 3, self.nc * self.ncg ** 2
         self.dof = 5 * self.K * self.np
         self.params = dict(gamma=gamma, re=re, pr=pr, mach=mach, shock=shock, precision=precision)
+        self.les = False            # static Smagorinsky model (set_les)
 
     # ------------------------------------------------------------------ util
     def _check(self, rc: int):
@@ -206,6 +207,7 @@ class DSEM:
     def set_les(self, enable: bool = True, cs: float = 0.17, numax: float = 1.0) -> None:
         """Static Smagorinsky model (staticSmag.f90): nu_t = Re (C_s Delta)^2 |S|."""
         self._check(self._L.dsem_set_les(self._h, int(enable), cs, numax))
+        self.les = bool(enable)
 
     def set_les_sensors(self, shock_sensor: bool = False, rho_sensor: bool = False) -> None:
         """Smagorinsky sensor switches (shockSensor / rhoSensor, staticSmag.f90:68-69)."""
@@ -294,9 +296,10 @@ class DSEM:
         return nb.value
 
     def tape_reserve(self, nslots: int) -> None:
-        if nslots > getattr(self, "_tape_slots", 0):
-            self.tape_owner = None  # a larger tape is a new allocation
-            self._tape_slots = nslots
+        if nslots > getattr(self, "_tape_slots", 0) or self.les != getattr(self, "_tape_les", False):
+            self.tape_owner = None  # a larger tape, or one laid out for nu_t, is a new allocation
+            self._tape_slots = max(nslots, getattr(self, "_tape_slots", 0))
+            self._tape_les = self.les
         self._check(self._L.dsem_tape_reserve(self._h, nslots))
 
     def rk_step_record(self, dt: float, slot: int) -> None:

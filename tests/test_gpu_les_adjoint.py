@@ -1,0 +1,127 @@
+"""Adjoint of the static Smagorinsky path (no counterpart in the reference,
+which has no derivative code) against central differences.
+
+tstep computes nu_t once per step from the step's start state and the
+lagged staggered states the previous step's last stage left (simpleSmag,
+staticSmag.f90:32-103; filteredStrain :126-179; nut_prolong :206-261), and
+the viscous fluxes read amu = 1 + nu_t rho (+ mu_h) (Vis_Fluxes_new.f90:123-125).
+A run through torch.autograd (RKSolve carrying the lagged states and their
+adjoint across steps; nu_t's adjoint through k_volume's reverse, k_smag's
+reverse and its gather) is checked by dJ/dJQ0 of J = int 1/2 rho |u|^2 after
+4 steps along seeded directions and by dJ/dnu (nu_t = Re (C_s Delta)^2 S is
+itself a function of nu = 1/Re), with the numax cap off and binding at part of
+the points, on distorted and rotated meshes (p = 3 on the scalar volume
+reverse, p >= 4 on its matrix-core form), alone and with entropy viscosity.
+Two checkpoint schedules must give the same gradient.  Where the cap binds,
+nu_t has kinks, and a central difference whose step straddles one at some
+point is off there: those cases take steps of 1e-7 (on the rotated mesh one
+direction measured 1.0e-6 relative at 1e-6, 3.4e-8 at 1e-7; without the cap
+both give 1e-9).
+"""
+import numpy as np
+import pytest
+
+from dsem_amd import cases
+from dsem_amd import mesh as M
+from dsem_amd.solver import DSEM
+from helpers import distorted_box
+
+pytestmark = pytest.mark.gpu
+
+CS = 0.12
+
+
+def _rel(a, b):
+    return abs(a - b) / max(abs(b), 1e-300)
+
+
+CASES = [
+    ("distorted p=3 no cap", lambda: distorted_box(3, 3, amp=0.04, seed=2), 1e9, {}),
+    ("distorted p=5", lambda: distorted_box(3, 5, amp=0.04, seed=5), 0.3, {}),
+    ("rotated p=4", lambda: M.rotated_box(4, 4, seed=4), 0.3, {}),
+    ("distorted p=4 with entropy viscosity", lambda: distorted_box(3, 4, amp=0.04, seed=3), 0.3,
+     dict(shock=True, cmu=2.0, mumax=0.05, ducros=True)),
+]
+
+
+@pytest.mark.parametrize("name,make_mesh,numax,extra", CASES, ids=[c[0] for c in CASES])
+def test_les_run_gradient_through_autograd(name, make_mesh, numax, extra):
+    import torch
+    from dsem_amd import autograd as A
+    print(f"\n[case] Smagorinsky run adjoint, {name}, 4 steps", flush=True)
+    mesh = make_mesh()
+    kw = dict(re=200.0, **extra)
+    nsteps = 4
+    with DSEM(mesh, **kw) as s:
+        s.set_les(True, CS, numax)
+        xg, jac = s.gauss_points()
+        JQ = cases.tgv_state(xg, jac, "isothermal", perturb=3e-2, seed=4)
+        s.set_state(JQ)
+        dt = 0.3 * s.dtcalc()
+        s.tstep(0.0, dt)
+        ng, _ = s.les_update()
+        print(f"nu_t after one step: max {ng.max():.3e}, share at the numax cap {np.mean(ng >= numax):.3f}, "
+              f"share at zero {np.mean(ng <= 0.0):.3f}", flush=True)
+        assert np.mean(ng >= numax) < 1.0
+        s.set_state(JQ)
+        bmat = A.bmat_tensor(s, "cuda")
+        grads = {}
+        for snaps, tape in ((None, None), (1, 1)):
+            x = torch.tensor(JQ, device="cuda", requires_grad=True)
+            nu = torch.tensor(1.0 / 200.0, dtype=torch.float64, device="cuda", requires_grad=True)
+            J = A.kinetic_energy_integral(A.solve(s, x, nsteps, dt, every=2, nu=nu, snaps=snaps, tape=tape), bmat)
+            J.backward()
+            grads[(snaps, tape)] = (x.grad.cpu().numpy(), float(nu.grad))
+        g, gnu = grads[(None, None)]
+        g2, gnu2 = grads[(1, 1)]
+        assert np.linalg.norm(g2 - g) <= 1e-13 * np.linalg.norm(g) and abs(gnu2 - gnu) <= 1e-12 * abs(gnu)
+        bm = s.quadrature_weights()
+
+        def Jf(q0, ctx=s):
+            ctx.set_state(q0)
+            ctx.steps(nsteps, dt, check=False)
+            q = ctx.get_state()
+            return 0.5 * float(np.sum(bm * (q[:, 1] ** 2 + q[:, 2] ** 2 + q[:, 3] ** 2) / q[:, 0]))
+
+        rng = np.random.default_rng(2025)
+        h = 1e-6 if numax > 1e6 else 1e-7
+        for trial in range(3):
+            d = rng.standard_normal(JQ.shape) * np.abs(JQ).mean(axis=(0, 2), keepdims=True)
+            fd = (Jf(JQ + h * d) - Jf(JQ - h * d)) / (2 * h)
+            ad = float(np.sum(g * d))
+            print(f"trial {trial}: adjoint {ad:.12e} fd {fd:.12e} rel {_rel(ad, fd):.2e}", flush=True)
+            assert _rel(ad, fd) < 1e-6
+    nu0 = 1.0 / 200.0
+    dnu = nu0 * (1e-4 if numax > 1e6 else 1e-6)
+    vals = []
+    for nu_s in (nu0 + dnu, nu0 - dnu):
+        kw2 = dict(kw)
+        kw2["re"] = 1.0 / nu_s
+        with DSEM(mesh, **kw2) as s2:
+            s2.set_les(True, CS, numax)
+            vals.append(Jf(JQ, s2))
+    fdn = (vals[0] - vals[1]) / (2 * dnu)
+    print(f"nu: adjoint {gnu:.12e} fd {fdn:.12e} rel {_rel(gnu, fdn):.2e}", flush=True)
+    assert _rel(gnu, fdn) < 1e-5
+
+
+def test_les_adjoint_refusals():
+    """The sensors' adjoint is not built; the plain entry points refuse the
+    model; a tape reserved before switching the model on is laid out again."""
+    from dsem_amd import _lib as L
+    print("\n[case] Smagorinsky adjoint refusals 2^3 p=3", flush=True)
+    mesh = distorted_box(2, 3, amp=0.04, seed=1)
+    with DSEM(mesh, re=200.0) as s:
+        xg, jac = s.gauss_points()
+        s.set_state(cases.tgv_state(xg, jac, "isothermal", perturb=3e-2, seed=1))
+        s.tape_reserve(2)
+        s.set_les(True, CS, 0.3)
+        with pytest.raises(L.DsemError):
+            s.rk_step_record(1e-3, 0)     # the tape holds no nu_t yet
+        s.tape_reserve(2)
+        s.rk_step_record(1e-3, 0)
+        with pytest.raises(L.DsemError):
+            s.rhs_vjp(np.zeros((s.K, 5, s.np)))
+        s.set_les_sensors(shock_sensor=True)
+        with pytest.raises(L.DsemError):
+            s.tape_reserve(2)
