@@ -18,9 +18,10 @@ def _rel(a, b):
     return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
 
 
-def test_eddy_viscosity_matches_oracle():
-    print("\n[case] smagorinsky nu_t distorted 3^3 p=5", flush=True)
-    mesh = distorted_box(3, 5, amp=0.06, seed=6)
+@pytest.mark.parametrize("n,p", [(3, 1), (3, 5), (2, 8)])
+def test_eddy_viscosity_matches_oracle(n, p):
+    print(f"\n[case] smagorinsky nu_t distorted {n}^3 p={p}", flush=True)
+    mesh = distorted_box(n, p, amp=0.06, seed=6)
     g = O.Geometry(mesh)
     JQ = cases.tgv_state(g.xg, g.jacob, "isothermal", perturb=2e-2, seed=5)
     ph = O.phys(re=800.0)
@@ -37,9 +38,11 @@ def test_eddy_viscosity_matches_oracle():
     assert _rel(ng, ng_r) < 1e-12 and _rel(ns, ns_r) < 1e-12
 
 
-def test_rhs_with_eddy_viscosity_matches_oracle():
-    print("\n[case] rhs with nu_t 3^3 p=4", flush=True)
-    mesh = M.periodic_box(3, 4)
+@pytest.mark.parametrize("n,p", [(3, 4), (2, 8)])
+def test_rhs_with_eddy_viscosity_matches_oracle(n, p):
+    """p = 4 and p = 8 (1024 work-items per element, 151 KB of LDS)."""
+    print(f"\n[case] rhs with nu_t {n}^3 p={p}", flush=True)
+    mesh = M.periodic_box(n, p)
     g = O.Geometry(mesh)
     JQ = cases.tgv_state(g.xg, g.jacob, "isothermal", perturb=1e-2, seed=2)
     ph = O.phys(re=600.0)

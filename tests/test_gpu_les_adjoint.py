@@ -37,6 +37,7 @@ def _rel(a, b):
 
 CASES = [
     ("distorted p=3 no cap", lambda: distorted_box(3, 3, amp=0.04, seed=2), 1e9, {}),
+    ("distorted 2^3 p=8 no cap", lambda: distorted_box(2, 8, amp=0.04, seed=8), 1e9, {}),
     ("distorted p=5", lambda: distorted_box(3, 5, amp=0.04, seed=5), 0.3, {}),
     ("rotated p=4", lambda: M.rotated_box(4, 4, seed=4), 0.3, {}),
     ("distorted p=4 with entropy viscosity", lambda: distorted_box(3, 4, amp=0.04, seed=3), 0.3,
