@@ -12,7 +12,9 @@ reverse and its gather) is checked by dJ/dJQ0 of J = int 1/2 rho |u|^2 after
 itself a function of nu = 1/Re), with the numax cap off and binding at part of
 the points, on distorted and rotated meshes (p = 3 on the scalar volume
 reverse, p >= 4 on its matrix-core form), alone and with entropy viscosity.
-Two checkpoint schedules must give the same gradient.  Where the cap binds,
+Two checkpoint schedules must give the same gradient.  The gradients of
+Smagorinsky and entropy-viscosity runs are also checked against central
+differences through the CPU oracle's steps.  Where the cap binds,
 nu_t has kinks, and a central difference whose step straddles one at some
 point is off there: those cases take steps of 1e-7 (on the rotated mesh one
 direction measured 1.0e-6 relative at 1e-6, 3.4e-8 at 1e-7; without the cap
@@ -126,3 +128,55 @@ def test_les_adjoint_refusals():
         s.set_les_sensors(shock_sensor=True)
         with pytest.raises(L.DsemError):
             s.tape_reserve(2)
+
+
+@pytest.mark.parametrize("model", ["smagorinsky", "entropy viscosity"])
+def test_lagged_model_gradient_against_oracle_finite_differences(model):
+    """dJ/dJQ0 of a 4-step run with the Smagorinsky model (cap off) or with
+    entropy viscosity (Ducros sensor and smoothing on, mumax cap off), from
+    the GPU adjoint, against central differences through the CPU oracle's
+    steps (or_rk_step_les / or_rk_step_ev: the restated reference, lagged
+    states carried from step to step, the first step a fresh start)."""
+    import torch
+    import oracle as O
+    from dsem_amd import autograd as A
+    print(f"\n[case] {model} adjoint vs oracle FD 2^3 p=4, 4 steps", flush=True)
+    mesh = M.periodic_box(2, 4)
+    g = O.Geometry(mesh)
+    JQ = cases.tgv_state(g.xg, g.jacob, "isothermal", perturb=3e-2, seed=7)
+    ph = O.phys(re=400.0)
+    dt = 0.4 * O.dtcalc(g, JQ, ph)
+    nsteps = 4
+    les = model == "smagorinsky"
+    kw = dict(re=400.0, metrics="per_point")
+    if not les:
+        kw.update(shock=True, cmu=2.0, mumax=1e9, ducros=True, smoothing=True)
+    with DSEM(mesh, **kw) as s:
+        if les:
+            s.set_les(True, CS, 1e9)
+        bmat = A.bmat_tensor(s, "cuda")
+        x = torch.tensor(JQ, device="cuda", requires_grad=True)
+        J = A.kinetic_energy_integral(A.solve(s, x, nsteps, dt, every=2), bmat)
+        J.backward()
+        grad = x.grad.cpu().numpy()
+    ev = O.OrEv(2.0, 1e9, 1, 1)
+    bm = O.ke_weights(mesh.p)
+
+    def Jf(q0):
+        q = q0.copy()
+        stag = np.zeros((mesh.K, 3, 5, g.ns))
+        for k in range(nsteps):
+            if les:
+                O.rk_step_les(g, q, dt, stag, k > 0, CS, 1e9, ph)
+            else:
+                O.rk_step_ev(g, q, dt, stag, k > 0, ev, ph)
+        return 0.5 * float(np.sum(bm * (q[:, 1] ** 2 + q[:, 2] ** 2 + q[:, 3] ** 2) / q[:, 0]))
+
+    rng = np.random.default_rng(99)
+    h = 1e-6
+    for trial in range(2):
+        d = rng.standard_normal(JQ.shape) * np.abs(JQ).mean(axis=(0, 2), keepdims=True)
+        fd = (Jf(JQ + h * d) - Jf(JQ - h * d)) / (2 * h)
+        ad = float(np.sum(grad * d))
+        print(f"trial {trial}: adjoint {ad:.12e} oracle fd {fd:.12e} rel {_rel(ad, fd):.2e}", flush=True)
+        assert _rel(ad, fd) < 1e-6
