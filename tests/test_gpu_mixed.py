@@ -1,4 +1,5 @@
-"""Mixed-precision contraction modes against the FP64 GPU path.
+"""Mixed-precision contraction modes against the FP64 GPU path, and one RHS
+of every mode against the CPU oracle (per-field L2 and Linf).
 
 FP32 contractions on element-shifted fields: one RHS of the throughput input
 (TGV with a seeded 1e-3 relative perturbation) within 1e-6 relative L2 per
@@ -62,6 +63,42 @@ def test_fp32_rhs_every_order(p):
         e = field_errors(s32.rhs(), s64.rhs())
     print({k: v["rel_l2"] for k, v in e.items()}, flush=True)
     assert all(v["rel_l2"] < 1e-6 for v in e.values())
+
+
+_ORACLE_RHS = {}
+
+
+def _oracle_case(n):
+    """(mesh, state, oracle RHS) of the throughput input's kind at n^3, p = 8,
+    computed once per size."""
+    import oracle as O
+    if n not in _ORACLE_RHS:
+        mesh = M.periodic_box(n, 8)
+        g = O.Geometry(mesh)
+        JQ = cases.tgv_state(g.xg, g.jacob, "reference", perturb=1e-3, seed=1234)
+        _ORACLE_RHS.clear()
+        _ORACLE_RHS[n] = (mesh, JQ, O.rhs(g, JQ, O.phys(re=1600.0, pr=0.71)))
+    return _ORACLE_RHS[n]
+
+
+# (elements per direction, mode, relative L2 bound per field).  The FP32 error
+# grows with the element size: 3.9e-6 at 4^3 against 7.5e-7 at 22^3.
+ORACLE_CASES = [(22, "fp64", 1e-12), (22, "fp32", 1e-6), (22, "bf16x3", 2e-4), (4, "fp32", 1e-5)]
+
+
+@pytest.mark.parametrize("n,mode,bound", ORACLE_CASES)
+def test_mixed_rhs_against_oracle(n, mode, bound):
+    """Per-field relative L2 and Linf of one RHS against the CPU oracle (the
+    restated reference, its leaf routines bitwise equal to the compiled
+    reference ones): TGV with a seeded 1e-3 relative perturbation, p = 8,
+    per-point metrics, at the throughput size 22^3 and at 4^3."""
+    print(f"\n[case] {mode} rhs vs oracle {n}^3 p=8", flush=True)
+    mesh, JQ, ref = _oracle_case(n)
+    with DSEM(mesh, re=1600.0, pr=0.71, precision=mode, metrics="per_point") as s:
+        s.set_state(JQ)
+        e = field_errors(s.rhs(), ref)
+    print(" ".join(f"{k}: L2 {v['rel_l2']:.2e} Linf {v['rel_linf']:.2e}" for k, v in e.items()), flush=True)
+    assert all(v["rel_l2"] < bound for v in e.values())
 
 
 def test_mixed_precision_is_rejected_for_adjoint():
