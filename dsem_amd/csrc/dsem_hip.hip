@@ -953,12 +953,23 @@ __device__ __forceinline__ void volume_inputs(const double *__restrict__ JQ, con
 //         Fbar_d * sum_{interior j} D[k][j] back in FP64.
 // Pointwise physics, the gradient metric combination inputs, the RK state
 // and all face kernels stay FP64.
-template <int P, int PM>
+// PLAIN: aligned faces, no entropy or eddy viscosity, no lagged states saved.
+// That instance compiles the optional paths out (at p = 8 FP64: 126 VGPRs
+// and no spill against 128 with a 12-byte spill, a quarter fewer VALU
+// instructions in the code); the other handles every case by run-time
+// branches.
+template <int P, int PM, bool PLAIN>
 __global__ __launch_bounds__((vblock<P, PM>()), 4) void k_volume(const double *__restrict__ JQ, DevMesh m, DevPhys ph,
                                                     const double *__restrict__ QA, const double *__restrict__ muh,
                                                     const double *__restrict__ nut, double *__restrict__ FV,
                                                     double *__restrict__ Qtp, double *__restrict__ Slag)
 {
+    if constexpr (PLAIN) {
+        muh = nullptr;
+        nut = nullptr;
+        Slag = nullptr;
+        m.rot = 0;
+    }
     using S = Sz<P>;
     using CT = typename Ctr<PM>::T;
     using CA = typename Ctr<PM>::A;
@@ -2178,7 +2189,8 @@ int volume_grid_size(dsem_ctx *c, int &grid)
     int &g = c->vol_grid[PM];
     if (g == 0) {
         int per_cu = 0, ncu = 0;
-        HIP_TRY(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_volume<P, PM>, vblock<P, PM>(), 0));
+        // both instances hold the same LDS, which allows one workgroup per CU
+        HIP_TRY(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_volume<P, PM, false>, vblock<P, PM>(), 0));
         HIP_TRY(c, hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->device));
         if (per_cu < 1 || ncu < 1) return set_err(c, DSEM_ERR_HIP, "volume kernel: no resident work-group fits");
         g = per_cu * ncu;
@@ -2222,13 +2234,21 @@ int run_rhs(dsem_ctx *c, double *JQ, const StageArgs &st, double *Qt_out, bool t
     if (c->params.precision == DSEM_PREC_FP32) TRY((volume_grid_size<P, 1>(c, gv1)));
     else if (c->params.precision == DSEM_PREC_BF16X3) TRY((volume_grid_size<P, 2>(c, gv2)));
     else TRY((volume_grid_size<P, 0>(c, gv0)));
+    const bool plain = !c->rotated && !muh && !nut && !slag;
+    auto vol = [&](auto pm, auto pl, int grid) {
+        constexpr int PM = decltype(pm)::value;
+        k_volume<P, PM, decltype(pl)::value><<<grid, vblock<P, PM>(), 0, s>>>(JQ, m, ph, b.QA, muh, nut, b.FV, b.Qtp,
+                                                                             slag);
+    };
+    using T_ = std::true_type;
+    using F_ = std::false_type;
     TRY(launch(c, KID_VOLUME, [&] {
         if (c->params.precision == DSEM_PREC_FP32)
-            k_volume<P, 1><<<gv1, vblock<P, 1>(), 0, s>>>(JQ, m, ph, b.QA, muh, nut, b.FV, b.Qtp, slag);
+            plain ? vol(std::integral_constant<int, 1>{}, T_{}, gv1) : vol(std::integral_constant<int, 1>{}, F_{}, gv1);
         else if (c->params.precision == DSEM_PREC_BF16X3)
-            k_volume<P, 2><<<gv2, vblock<P, 2>(), 0, s>>>(JQ, m, ph, b.QA, muh, nut, b.FV, b.Qtp, slag);
+            plain ? vol(std::integral_constant<int, 2>{}, T_{}, gv2) : vol(std::integral_constant<int, 2>{}, F_{}, gv2);
         else
-            k_volume<P, 0><<<gv0, vblock<P, 0>(), 0, s>>>(JQ, m, ph, b.QA, muh, nut, b.FV, b.Qtp, slag);
+            plain ? vol(std::integral_constant<int, 0>{}, T_{}, gv0) : vol(std::integral_constant<int, 0>{}, F_{}, gv0);
     }));
     TRY(launch(c, KID_FACE_VISC, [&] { k_face_visc<P><<<face_blocks, BLOCK, 0, s>>>(b.FV, b.FI, m, ph, b.FT); }));
     if (make_traces)
