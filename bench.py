@@ -36,8 +36,8 @@ FP64_PEAK_TFLOPS = 78.6
 # FP64, per-point metrics): FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, the factors
 # that make the counters read 1 GiB for a 1 GiB device copy.  Source:
 # profiles/tgv22p8_fp64.md.
-MEASURED_TRAFFIC = {(22, 8, "fp64", "per_point"): {"k_volume": 2.533e9, "k_update": 2.014e9, "k_face_inv": 0.541e9,
-                                                   "k_face_visc": 0.478e9, "k_traces": 0.583e9}}
+MEASURED_TRAFFIC = {(22, 8, "fp64", "per_point"): {"k_volume": 2.449e9, "k_update": 2.015e9, "k_face_inv": 0.541e9,
+                                                   "k_face_visc": 0.477e9, "k_traces": 0.583e9}}
 
 
 def kernel_work(n_elem: int, nmort: int, p: int) -> dict:

@@ -2456,14 +2456,17 @@ int run_rhs_vjp(dsem_ctx *c, double *JQ, FwdTape f, bool fwd, double *Gb, double
     return launch(c, KID_ADJ, [&] {
         k_adj_update<P><<<K, BLOCK, 0, s>>>(Gb, Qb, cdt, c->aFTbe);
         k_adj_face_visc<P><<<face_blocks, BLOCK, 0, s>>>(c->aFTbe, f.FV, m, ph, c->aFIb, c->aFVb, nu_f);
-        auto vol = [&](auto rot, auto les) {
-            k_adj_volume<P, (P >= 4), decltype(rot)::value, decltype(les)::value><<<K, ABLOCK, 0, s>>>(
+        auto vol = [&](auto rot, auto lag) {
+            k_adj_volume<P, (P >= 4), decltype(rot)::value, decltype(lag)::value><<<K, ABLOCK, 0, s>>>(
                 JQ, m, ph, f.QA, muh, Gb, c->aFVb, c->aQVb, c->aGVb, nu_v, MuB, SlagB, nut, NutB);
         };
-        using T_ = std::true_type;
-        using F_ = std::false_type;
-        if (c->rotated) nut ? vol(T_{}, T_{}) : vol(T_{}, F_{});
-        else nut ? vol(F_{}, T_{}) : vol(F_{}, F_{});
+        auto lagged = [&](auto rot) {
+            if (nut) vol(rot, std::integral_constant<int, 2>{});
+            else if (muh || MuB || SlagB) vol(rot, std::integral_constant<int, 1>{});
+            else vol(rot, std::integral_constant<int, 0>{});
+        };
+        if (c->rotated) lagged(std::true_type{});
+        else lagged(std::false_type{});
         if constexpr (P >= 5) k_adj_gather_dma<P><<<K, GBLOCK, 0, s>>>(m, c->aQVb, c->aGVb, c->aJQvb, c->aQAbe);
         else k_adj_gather<P><<<K, GBLOCK, 0, s>>>(m, c->aQVb, c->aGVb, c->aJQvb, c->aQAbe);
         k_adj_face_inv<P><<<face_blocks, BLOCK, 0, s>>>(f.T, m, ph, c->aFIb, c->aQAbe, c->aTb);
