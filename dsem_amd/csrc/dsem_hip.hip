@@ -957,7 +957,8 @@ __device__ __forceinline__ void volume_inputs(const double *__restrict__ JQ, con
 // That instance compiles the optional paths out (at p = 8 FP64: 126 VGPRs
 // and no spill against 128 with a 12-byte spill, a quarter fewer VALU
 // instructions in the code); the other handles every case by run-time
-// branches.
+// branches.  The mixed modes use the general instance only: their plain one
+// spilled 28 bytes at p = 8 and ran slower.
 template <int P, int PM, bool PLAIN>
 __global__ __launch_bounds__((vblock<P, PM>()), 4) void k_volume(const double *__restrict__ JQ, DevMesh m, DevPhys ph,
                                                     const double *__restrict__ QA, const double *__restrict__ muh,
@@ -2244,9 +2245,9 @@ int run_rhs(dsem_ctx *c, double *JQ, const StageArgs &st, double *Qt_out, bool t
     using F_ = std::false_type;
     TRY(launch(c, KID_VOLUME, [&] {
         if (c->params.precision == DSEM_PREC_FP32)
-            plain ? vol(std::integral_constant<int, 1>{}, T_{}, gv1) : vol(std::integral_constant<int, 1>{}, F_{}, gv1);
+            vol(std::integral_constant<int, 1>{}, F_{}, gv1);
         else if (c->params.precision == DSEM_PREC_BF16X3)
-            plain ? vol(std::integral_constant<int, 2>{}, T_{}, gv2) : vol(std::integral_constant<int, 2>{}, F_{}, gv2);
+            vol(std::integral_constant<int, 2>{}, F_{}, gv2);
         else
             plain ? vol(std::integral_constant<int, 0>{}, T_{}, gv0) : vol(std::integral_constant<int, 0>{}, F_{}, gv0);
     }));
