@@ -2070,6 +2070,7 @@ struct dsem_ctx {
     // the Smagorinsky model (with the constants in tape_cs, tape_numax)
     std::vector<char> tape_lag;
     std::vector<double> tape_cs, tape_numax;
+    std::vector<int> tape_sens;                // per tape slot: the Smagorinsky sensor switches
     bool tape_les = false;                     // the tape's slots hold nu_t (the layout of tape_reserve)
     // step tape: per slot and stage the stage input state and the forward
     // intermediates the adjoint reads (face traces T, mortar averages QA,
@@ -2514,26 +2515,29 @@ int ev_vjp_dispatch(dsem_ctx *c, const double *JQ, const double *Slag, const dou
     DISPATCH_P(c, run_ev_vjp, JQ, Slag, MuB, JQb, SlagB);
 }
 
-// Adjoint of the Smagorinsky update (k_smag, no sensors) at state JQ and
-// lagged states Slag: JQb += (d nu_t / d JQ)^T NutB, SlagB += (d nu_t /
-// d Slag)^T NutB, nu_slot = <NutB, d nu_t / d nu> (nu_t is linear in Re).
+// Adjoint of the Smagorinsky update (k_smag, with its sensor switches) at
+// state JQ and lagged states Slag: JQb += (d nu_t / d JQ)^T NutB, SlagB +=
+// (d nu_t / d Slag)^T NutB, nu_slot = <NutB, d nu_t / d nu> (nu_t is linear
+// in Re).
 template <int P>
-int run_les_vjp(dsem_ctx *c, const double *JQ, const double *Slag, double cs, double numax, const double *NutB,
-                double *JQb, double *SlagB, double *nu_slot)
+int run_les_vjp(dsem_ctx *c, const double *JQ, const double *Slag, double cs, double numax, int sensors,
+                const double *NutB, double *JQb, double *SlagB, double *nu_slot)
 {
     const DevMesh m = dev_mesh(c);
     const DevPhys ph = dev_phys(c);
     hipStream_t s = c->stream;
+    const int ev_on = c->params.shock ? 1 : 0;
     return launch(c, KID_ADJ, [&] {
-        k_adj_smag<P><<<c->K, BLOCK, 0, s>>>(JQ, Slag, m, ph, cs, numax, NutB, JQb, c->aWB, c->aNuPart);
+        k_adj_smag<P><<<c->K, BLOCK, 0, s>>>(JQ, Slag, m, ph, cs, numax, sensors, ev_on, NutB, JQb, c->aWB,
+                                             c->aNuPart);
         k_adj_smag_gather<P><<<c->K, BLOCK, 0, s>>>(c->aWB, SlagB, c->aNuPart, c->K, nu_slot);
     });
 }
 
-int les_vjp_dispatch(dsem_ctx *c, const double *JQ, const double *Slag, double cs, double numax, const double *NutB,
-                     double *JQb, double *SlagB, double *nu_slot)
+int les_vjp_dispatch(dsem_ctx *c, const double *JQ, const double *Slag, double cs, double numax, int sensors,
+                     const double *NutB, double *JQb, double *SlagB, double *nu_slot)
 {
-    DISPATCH_P(c, run_les_vjp, JQ, Slag, cs, numax, NutB, JQb, SlagB, nu_slot);
+    DISPATCH_P(c, run_les_vjp, JQ, Slag, cs, numax, sensors, NutB, JQb, SlagB, nu_slot);
 }
 
 // Tape layout per slot and stage: state [n], T [nT], QA [nQA], FV [nT].
@@ -2599,6 +2603,7 @@ int tape_reserve(dsem_ctx *c, int nslots)
     c->tape_lag.assign(nslots, 0);
     c->tape_cs.assign(nslots, 0.0);
     c->tape_numax.assign(nslots, 0.0);
+    c->tape_sens.assign(nslots, 0);
     return DSEM_OK;
 }
 
@@ -2623,6 +2628,7 @@ int record_step(dsem_ctx *c, double dt, int slot)
         c->tape_lag[slot] = (c->have_lag ? 1 : 0) | (c->les ? 2 : 0);
         c->tape_cs[slot] = c->les_cs;
         c->tape_numax[slot] = c->les_numax;
+        c->tape_sens[slot] = c->les_sensors;
         if (c->have_lag)
             HIP_TRY(c, hipMemcpyAsync(ts, c->Slag, sizeof(double) * (size_t)c->K * 3 * NEQ * c->ns,
                                       hipMemcpyDeviceToDevice, c->stream));
@@ -2701,7 +2707,8 @@ int vjp_recorded_step(dsem_ctx *c, double dt, int slot, double *dQbar, double *n
         if (ev) TRY(ev_vjp_dispatch(c, S0, ts, c->aMuB, dQbar, slb));   // sets slb
         else HIP_TRY(c, hipMemsetAsync(slb, 0, sizeof(double) * nm * NEQ, c->stream));
         if (les)
-            TRY(les_vjp_dispatch(c, S0, ts, c->tape_cs[slot], c->tape_numax[slot], c->aNutB, dQbar, slb, c->aNu + 6));
+            TRY(les_vjp_dispatch(c, S0, ts, c->tape_cs[slot], c->tape_numax[slot], c->tape_sens[slot], c->aNutB, dQbar,
+                                 slb, c->aNu + 6));
     } else if (dSlagbar) {
         // a fresh start: mu_h = nu_t = 0 whatever the state
         HIP_TRY(c, hipMemsetAsync(dSlagbar, 0, sizeof(double) * nm * NEQ, c->stream));
@@ -3324,8 +3331,6 @@ static int vjp_check(dsem_ctx *c, bool allow_lagged = false)
         return set_err(c, DSEM_ERR_UNSUPPORTED,
                        "with entropy viscosity or the Smagorinsky model use the tape entry points "
                        "(dsem_rk_step_record, dsem_rk_step_vjp_tape_lagged) or dsem_ev_vjp");
-    if (c->les && c->les_sensors)
-        return set_err(c, DSEM_ERR_UNSUPPORTED, "the adjoint does not cover the Smagorinsky sensors");
     if (c->params.precision != DSEM_PREC_FP64)
         return set_err(c, DSEM_ERR_UNSUPPORTED, "the adjoint is built for the FP64 path");
     HIP_TRY(c, hipSetDevice(c->device));

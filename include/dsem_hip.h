@@ -230,14 +230,15 @@ int  dsem_rk_step_vjp_tape(dsem_ctx *ctx, double dt, int slot, double *dQbar, do
  *   (dsem_rk_step_record keeps its mu_h and the lagged states it read);
  *   dQbar and dSlagbar (device) hold on entry the adjoints of the state and
  *   of the lagged states after the step, on return those before it.
- *   The same holds with the Smagorinsky model (dsem_set_les, sensors off),
- *   whose nu_t tstep also computes once per step from the start state and
- *   the lagged states (simpleSmag, staticSmag.f90:32-103): the slot keeps
- *   nu_t, and its adjoint goes back through k_smag's (clamp and prolongation
- *   zeroing take the forward's branch); nubar includes nu_t's own
- *   dependence on nu = 1/Re (nu_t = Re (C_s Delta)^2 |S|).  The tape's layout follows the
- *   Smagorinsky switch at dsem_tape_reserve: after switching, reserve again
- *   before recording.
+ *   The same holds with the Smagorinsky model (dsem_set_les, sensor
+ *   switches included), whose nu_t tstep also computes once per step from
+ *   the start state and the lagged states (simpleSmag,
+ *   staticSmag.f90:32-103): the slot keeps nu_t, and its adjoint goes back
+ *   through k_smag's (clamp and prolongation zeroing take the forward's
+ *   branch; the sensors are recomputed from the same states); nubar includes
+ *   nu_t's own dependence on nu = 1/Re (nu_t = Re (C_s Delta)^2 |S|).  The
+ *   tape's layout follows the Smagorinsky switch at dsem_tape_reserve: after
+ *   switching, reserve again before recording.
  * dsem_get/set_lagged_states_device: device copies for checkpoints; valid
  *   receives whether the context holds lagged states (a fresh start does
  *   not); set with NULL makes the next step a fresh start (mu_h = 0). */

@@ -109,8 +109,8 @@ def test_les_run_gradient_through_autograd(name, make_mesh, numax, extra):
 
 
 def test_les_adjoint_refusals():
-    """The sensors' adjoint is not built; the plain entry points refuse the
-    model; a tape reserved before switching the model on is laid out again."""
+    """The plain entry points refuse the model; a tape reserved before
+    switching the model on is laid out again."""
     from dsem_amd import _lib as L
     print("\n[case] Smagorinsky adjoint refusals 2^3 p=3", flush=True)
     mesh = distorted_box(2, 3, amp=0.04, seed=1)
@@ -125,9 +125,6 @@ def test_les_adjoint_refusals():
         s.rk_step_record(1e-3, 0)
         with pytest.raises(L.DsemError):
             s.rhs_vjp(np.zeros((s.K, 5, s.np)))
-        s.set_les_sensors(shock_sensor=True)
-        with pytest.raises(L.DsemError):
-            s.tape_reserve(2)
 
 
 @pytest.mark.parametrize("model", ["smagorinsky", "entropy viscosity"])
@@ -180,3 +177,91 @@ def test_lagged_model_gradient_against_oracle_finite_differences(model):
         ad = float(np.sum(grad * d))
         print(f"trial {trial}: adjoint {ad:.12e} oracle fd {fd:.12e} rel {_rel(ad, fd):.2e}", flush=True)
         assert _rel(ad, fd) < 1e-6
+
+
+SENSOR_CASES = [
+    ("shock sensor, entropy viscosity", dict(shock_sensor=True), True),
+    ("density sensor, entropy viscosity", dict(rho_sensor=True), True),
+    ("both sensors, entropy viscosity", dict(shock_sensor=True, rho_sensor=True), True),
+    ("both sensors, no entropy viscosity", dict(shock_sensor=True, rho_sensor=True), False),
+]
+
+
+@pytest.mark.parametrize("name,sensors,ev", SENSOR_CASES, ids=[c[0] for c in SENSOR_CASES])
+def test_les_sensor_run_gradient_through_autograd(name, sensors, ev):
+    """The sensor switches (staticSmag.f90:68-69): nu_t times 1 - ducros
+    and/or the density sensor, both k_ev's functions of the step's start
+    state and lagged states.  4 steps on a distorted 3^3 p = 4 mesh, cap off:
+    with entropy viscosity, dJ/dJQ0 and dJ/dnu against central differences;
+    without it the sensors are the zero arrays nothing writes, the density
+    switch makes nu_t 0, and the gradient must equal the one without the
+    model (to 1e-13: the volume kernels then run other instances).  (That run's own central differences are off by up to
+    2.6e-6 at a step of 1e-6 and 7.7e-7 at 1e-7, the same with the model
+    off: kinks of the forward without entropy viscosity, not the adjoint.)"""
+    import torch
+    from dsem_amd import autograd as A
+    print(f"\n[case] Smagorinsky sensors adjoint, {name}, 3^3 p=4, 4 steps", flush=True)
+    mesh = distorted_box(3, 4, amp=0.04, seed=11)
+    kw = dict(re=200.0)
+    if ev:
+        kw.update(shock=True, cmu=2.0, mumax=1e9)
+    nsteps = 4
+
+    def make(re):
+        s = DSEM(mesh, **dict(kw, re=re))
+        s.set_les(True, CS, 1e9)
+        s.set_les_sensors(**sensors)
+        return s
+
+    def grad(s, JQ, dt, bmat):
+        x = torch.tensor(JQ, device="cuda", requires_grad=True)
+        nu = torch.tensor(1.0 / 200.0, dtype=torch.float64, device="cuda", requires_grad=True)
+        J = A.kinetic_energy_integral(A.solve(s, x, nsteps, dt, every=2, nu=nu), bmat)
+        J.backward()
+        return x.grad.cpu().numpy(), float(nu.grad)
+
+    with make(200.0) as s:
+        xg, jac = s.gauss_points()
+        JQ = cases.tgv_state(xg, jac, "isothermal", perturb=3e-2, seed=12)
+        s.set_state(JQ)
+        dt = 0.3 * s.dtcalc()
+        bmat = A.bmat_tensor(s, "cuda")
+        g, gnu = grad(s, JQ, dt, bmat)
+        # the sensors' share of the gradient, so that the check below has power
+        s.set_les_sensors()
+        g0, _ = grad(s, JQ, dt, bmat)
+        s.set_les_sensors(**sensors)
+        share = np.linalg.norm(g - g0) / np.linalg.norm(g)
+        print(f"|g - g(sensors off)| / |g| = {share:.3e}", flush=True)
+        if not ev:
+            s.set_les(False)
+            g1, gnu1 = grad(s, JQ, dt, bmat)
+            print(f"|g - g(model off)| = {np.abs(g - g1).max():.3e}, nu {abs(gnu - gnu1):.3e}", flush=True)
+            assert np.abs(g - g1).max() <= 1e-13 * np.abs(g).max() and abs(gnu - gnu1) <= 1e-13 * abs(gnu)
+            return
+        assert share > 1e-4
+        bm = s.quadrature_weights()
+
+        def Jf(q0, ctx=s):
+            ctx.set_state(q0)
+            ctx.steps(nsteps, dt, check=False)
+            q = ctx.get_state()
+            return 0.5 * float(np.sum(bm * (q[:, 1] ** 2 + q[:, 2] ** 2 + q[:, 3] ** 2) / q[:, 0]))
+
+        rng = np.random.default_rng(77)
+        h = 1e-6
+        for trial in range(3):
+            d = rng.standard_normal(JQ.shape) * np.abs(JQ).mean(axis=(0, 2), keepdims=True)
+            fd = (Jf(JQ + h * d) - Jf(JQ - h * d)) / (2 * h)
+            ad = float(np.sum(g * d))
+            print(f"trial {trial}: adjoint {ad:.12e} fd {fd:.12e} rel {_rel(ad, fd):.2e}", flush=True)
+            assert _rel(ad, fd) < 1e-6
+    nu0 = 1.0 / 200.0
+    dnu = nu0 * 1e-4
+    vals = []
+    for nu_s in (nu0 + dnu, nu0 - dnu):
+        with make(1.0 / nu_s) as s2:
+            vals.append(Jf(JQ, s2))
+    fdn = (vals[0] - vals[1]) / (2 * dnu)
+    print(f"nu: adjoint {gnu:.12e} fd {fdn:.12e} rel {_rel(gnu, fdn):.2e}", flush=True)
+    assert _rel(gnu, fdn) < 1e-5
