@@ -1,7 +1,7 @@
 """GPU parity: the HIP path through the C ABI against the CPU oracle.
 
 Tolerance for the FP64 path: relative L2 per conserved field <= 1e-12 on
-the RHS; the GPU uses fused multiply-adds and a
+the RHS (the parity bar of the FP64 path); the GPU uses fused multiply-adds and a
 different summation order in the contractions, so results are not bitwise.
 
 Metric modes.  With per-point metrics the library uses the reference's
